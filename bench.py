@@ -1,0 +1,199 @@
+"""Throughput of the hot path on one MI355X: particle-steps/s at N = 2,097,152.
+
+Workload (the large memory-bound configuration): N = 2,097,152 colloids
+(radius 0.5) at volume fraction 0.30 in a periodic cube of edge
+L = 154.112, screened Yukawa repulsion (inverse Debye length 5, contact
+strength 20 kT -> yukawaStrength 14841.3 with debyeLength 0.2) plus the LJ
+2n-n core (ljNum 18, wellDepth 1), cutoff 3, 51^3 cells of edge 3.022,
+G&B integrator dt = 1e-3, gamma = 1, kT = 1, in-kernel Philox noise.
+One step = forces + integration + cell rebuild (the system::run loop body).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line.  The state (positions, velocities, noise generator)
+is resident in HBM before the timed region; the timed region is K steps
+bracketed by stream synchronisations.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 2097152
+PHI = 0.30
+CUTOFF = 3.0
+DEBYE = 0.2
+YUK_A = 20.0 / (DEBYE * math.exp(-1.0 / DEBYE))
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per particle of the fused step kernel (k_step):
+# read x, x0, F_prev, memX (4 x 24 B), write x', x0', F, memX' (4 x 24 B),
+# write cell key + atomic rank (2 x 4 B)
+STEP_KERNEL_BYTES = 200.0
+# algorithmic bytes per particle-step of the whole step (cell build and the
+# sorted position copy included), in-kernel noise:
+# 192 + 16 + 48 + 8 * ncell / N
+def step_bytes(ncell: int) -> float:
+    return 192.0 + 16.0 + 48.0 + 8.0 * ncell / N
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    return ap.parse_args()
+
+
+def cpu_baseline(pos_exact, box_exact, steps):
+    """The reference's own code (oracle/_ref, compiled from its sources) on
+    the host cores, in the reference's integer geometry for this workload
+    (boxSize 153, cellSize 3, scale 51); the oracle restatement if the
+    reference build is absent."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    from oracle import reference_available
+    box_i, cell_i, scale_i = 153, 3, 51
+    pos = np.ascontiguousarray(pos_exact * (box_i / box_exact))
+    pos = np.minimum(pos, np.nextafter(box_i, 0))
+    vel = np.random.default_rng(7).standard_normal((N, 3))
+    cfg = {"cutOff": CUTOFF, "kT": 1.0, "wellDepth": 1.0, "yukawaStrength": YUK_A,
+           "ljNum": 18, "debyeLength": DEBYE, "nParticles": N, "gamma": 1.0, "mass": 1.0,
+           "timeStep": 0.001, "velFreq": 1000, "seed": 1}
+    sample = (f"N={N}, reference integer geometry L={box_i} cell={cell_i} S={scale_i}, "
+              f"{steps} steps incl. the first, OpenMP threads={threads}")
+    if reference_available():
+        from oracle import Reference
+        secs = Reference().run("LJ", cfg, pos, vel, 0.5, 1.0, box_i, cell_i, scale_i, 0.001,
+                               steps, nthreads=threads, record=False)
+        kind = "reference"
+    else:
+        from oracle import ForceParams, load_oracle
+        o = load_oracle()
+        fp = ForceParams.make(2, CUTOFF, kT=1.0, lj_well_depth=1.0, yukawa_strength=YUK_A,
+                              debye_length=DEBYE, lj_num=18)
+        c = o.gb_coeffs(1.0, 1.0, 1.0, 1e-3)
+        st = dict(pos=pos, pos0=np.zeros((N, 3)), vel=vel, frc=np.zeros((N, 3)),
+                  frc0=np.zeros((N, 3)), memx=np.zeros((N, 3)))
+        t0 = time.perf_counter()
+        o.run(fp, c, st, steps, float(box_i), scale_i, float(cell_i), seed=1,
+              nthreads=threads)
+        secs = time.perf_counter() - t0
+        kind = "port"
+    return {"value": N * steps / secs, "unit": "particle-steps/s", "cores": threads,
+            "kind": kind, "sample": sample, "seconds": secs}
+
+
+def traffic_from_profiles():
+    """HBM bytes per k_step launch measured by rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE) and calibrated against a copy kernel; null when absent."""
+    path = os.path.join(ROOT, "profiles", "k_step_c3_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+    import proteinsim2015_amd as ps
+    geom = ps.SystemGeometry.exact(N, PHI, CUTOFF)
+    t_setup = time.perf_counter()
+    pos = ps.rsa_positions(N, geom.box, 1234)
+    vel = np.random.default_rng(4321).standard_normal((N, 3))
+    lj = ps.LennardJones(kT=1.0, well_depth=1.0, yukawa_strength=YUK_A, lj_num=18,
+                         cutoff=CUTOFF, debye_length=DEBYE)
+    integ = ps.BrownianIntegrator(kT=1.0, mass=1.0, gamma=1.0, dt=1e-3, vel_freq=1000,
+                                  seed=90210)
+    sysg = ps.GpuSystem(geom, lj, integ)
+    sysg.set_state(pos, vel=vel)
+    setup_s = time.perf_counter() - t_setup
+    print(f"bench: setup {setup_s:.1f}s (RSA + upload), warmup {args.warmup}", file=sys.stderr,
+          flush=True)
+    if args.warmup:
+        sysg.step_device_noise(args.warmup, 0)
+    sysg.sync()
+    code = sysg.last_error()[0]
+    if code:
+        raise SystemExit(f"bench: physics error {code} during warm-up")
+    sysg.set_profiling(True)
+    sysg.kernel_times(reset=True)
+    sysg.sync()
+    t0 = time.perf_counter()
+    sysg.step_device_noise(args.steps, 0)
+    sysg.sync()
+    t1 = time.perf_counter()
+    kt = sysg.kernel_times(reset=True)
+    sysg.set_profiling(False)
+    code = sysg.last_error()[0]
+    if code:
+        raise SystemExit(f"bench: physics error {code} in the timed region")
+    elapsed = t1 - t0
+    value = N * args.steps / elapsed
+    step_ms = kt["force_ms"] / max(1, kt["force_launches"])
+    achieved = STEP_KERNEL_BYTES * N / (step_ms * 1e-3) / 1e9
+    wall_step_bytes = step_bytes(geom.ncell)
+    print(f"bench: {args.steps} steps in {elapsed:.3f}s; k_step {step_ms:.3f} ms, "
+          f"build {kt['build_ms'] / max(1, kt['build_launches']):.3f} ms", file=sys.stderr,
+          flush=True)
+    sysg.close()
+    cpu = None if args.no_cpu_baseline else cpu_baseline(pos, geom.box, args.cpu_steps)
+    line = {
+        "metric": "particle-steps/s at N=2M",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded random sequential addition positions, Gaussian velocities",
+        "config": {
+            "workload": "N=2,097,152 Yukawa+LJ core, phi=0.30, L=154.112, cutoff 3, 51^3 cells, "
+                        "dt=1e-3, gamma=1, kT=1, in-kernel Philox noise",
+            "n_particles": N, "volume_fraction": PHI, "box": geom.box,
+            "cells_per_dim": geom.cells_per_dim, "cutoff": CUTOFF,
+            "potential": "LJ 36-18 (wellDepth 1) + Yukawa (debyeLength 0.2, "
+                         "yukawaStrength 14841.3)",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_step (fused 27-cell forces + G&B + cell key)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(),
+            "bytes_per_particle": STEP_KERNEL_BYTES,
+            "avg_launch_ms": step_ms,
+            "whole_step_frac": wall_step_bytes * value / (HBM_PEAK_GBS * 1e9),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+/*
+ * psim_gpu.h -- C ABI of the MI355X (gfx950) ProteinSim2015 hot path.
+ *
+ * Plain C: pointers, sizes and PODs only.  Host arrays are caller-owned and
+ * borrowed for the duration of a call; device memory is owned by the context.
+ * Status codes: 0 = OK; the reference's own exit codes for physics errors
+ * (7703 particle out of box, 7704 overlap, 7705 non-finite AO force; see
+ * ClusteredCore/src/utilities/error.cpp:40-62, where the reference calls
+ * exit()); negative values for API/runtime errors (PSIM_E_*).
+ *
+ * One context per host thread (or per stream); calls on one context are not
+ * re-entrant.  AoS3 = n x 3 doubles (x,y,z) in ORIGINAL particle order.
+ *
+ * Which reference interface each entry point replaces:
+ *   psim_gpu_create        PSim::system::setSystemConstants + the force
+ *                          plugin / integrator constructors
+ *                          (system.cpp:190-230, LJPotential.cpp:29-62,
+ *                          AOPotential.cpp:33-59, Calibration.cpp:30-39,
+ *                          brownianIntegrator.cpp:27-99)
+ *   psim_gpu_set_state     particle setters + the constructor's first cell
+ *                          build (particle.cpp:61-99, system.cpp:259-266)
+ *   psim_gpu_step          nsteps iterations of system::run
+ *                          (system.cpp:350-374): forces
+ *                          (defaultForceManager.cpp:55-65 ->
+ *                          IForce::getAcceleration, IForce.h:35),
+ *                          pushParticleForce (systemHandling.cpp:87-92),
+ *                          IIntegrator::nextSystem (IIntegrator.h:44,
+ *                          brownianIntegrator.cpp:181-356), cell rebuild
+ *                          (systemHandling.cpp:33-85)
+ *   psim_gpu_forces        defaultForceManager::getAcceleration alone
+ *   psim_gpu_force_table   IForce::getAcceleration over the reference's own
+ *                          sortedParticles / particleHashIndex /
+ *                          cellStartEnd arrays (IForce.h:35), for the plugin
+ *                          adaptor's batch path
+ *   psim_gpu_get_state     particle getters (particle.h:82-240)
+ *   psim_gpu_last_error    error::throw* (error.cpp:40-62), as a code
+ *   psim_init_rsa          system::initParticles + initCheck
+ *                          (systemInit.cpp:31-160), O(N) with a cell grid
+ */
+#ifndef PSIM_GPU_H
+#define PSIM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSIM_GPU_ABI_VERSION 1
+
+/* force-module bits; a composite is the sum of the modules' forces */
+enum {
+    PSIM_FORCE_CAL = 1, /* Calibration soft shell, Calibration.cpp:75-81   */
+    PSIM_FORCE_LJ = 2,  /* Lennard-Jones 2n-n + Yukawa, LJPotential.cpp:97-109 */
+    PSIM_FORCE_AO = 4   /* Asakura-Oosawa + r^-38 core, AOPotential.cpp:95-109 */
+};
+
+enum {
+    PSIM_OK = 0,
+    PSIM_ERR_INIT = 7701,    /* initial configuration cannot be placed   */
+    PSIM_ERR_BOUNDS = 7703,  /* particle out of box after the single wrap */
+    PSIM_ERR_OVERLAP = 7704, /* pair closer than 0.8 (r_i + r_j)          */
+    PSIM_ERR_NAN = 7705,     /* non-finite AO pair force                  */
+    PSIM_E_ARG = -1,
+    PSIM_E_HIP = -2,
+    PSIM_E_NOMEM = -3,
+    PSIM_E_UNSUPPORTED = -4
+};
+
+typedef struct psim_params {
+    int n_particles;        /* nParticles                                   */
+    int cells_per_dim;      /* S, the reference's cellScale (>= 3)          */
+    double box;             /* L; an integer in the reference's geometry    */
+    double cell_size;       /* cell edge used by the hash (cellSize)        */
+    double half_uv;         /* half box used by unitVectorAdv; 0 -> integer
+                               L/2 when L is integral, else L/2             */
+    double radius;          /* particle radius (all particles)              */
+    double mass;
+    double kT;
+    double gamma;
+    double dt;              /* timeStep                                     */
+    int vel_freq;           /* velFreq                                      */
+    int force_mask;         /* PSIM_FORCE_* bits                            */
+    int lj_num;             /* ljNum                                        */
+    int reserved0;
+    double cutoff;          /* cutOff (shared by all modules)               */
+    double lj_well_depth;   /* wellDepth of the LJ module (epsilon)         */
+    double yukawa_strength; /* yukawaStrength                               */
+    double debye_length;    /* debyeLength                                  */
+    double ao_well_depth;   /* wellDepth of the AO module (phi_p)           */
+    unsigned long long seed;/* in-kernel noise seed                        */
+} psim_params;
+
+typedef struct psim_ctx psim_ctx;
+
+/* per-kernel device time accumulated since the last reset (HIP events on
+ * the context's stream), in milliseconds, with launch counts */
+typedef struct psim_kernel_times {
+    double force_ms;   int force_launches;
+    double build_ms;   int build_launches;   /* scan + scatter + reorder */
+    double total_ms;   int steps;
+} psim_kernel_times;
+
+int psim_gpu_abi_version(void);
+const char *psim_gpu_status_string(int status);
+
+int psim_gpu_create(const psim_params *params, psim_ctx **out);
+void psim_gpu_destroy(psim_ctx *ctx);
+
+/* Upload a state.  pos is required; pos0/vel/frc/frc0/memx may be NULL
+ * (zeros; pos0 NULL = the reference's fresh-particle image, i.e.
+ * safeMod0(0, x)).  current_time == 0 selects the first-step branch. */
+int psim_gpu_set_state(psim_ctx *ctx, const double *pos, const double *pos0,
+                       const double *vel, const double *frc, const double *frc0,
+                       const double *memx, double current_time, int vel_counter,
+                       unsigned long long step_index);
+int psim_gpu_get_state(psim_ctx *ctx, double *pos, double *pos0, double *vel,
+                       double *frc, double *frc0, double *memx,
+                       double *current_time, int *vel_counter,
+                       unsigned long long *step_index);
+
+/* Advance nsteps.  noise == NULL: in-kernel Philox4x32-10 normals keyed by
+ * (seed, step index, original id).  Otherwise host array [nsteps][6][n] by
+ * original index (A_x,A_y,A_z,B_x,B_y,B_z; the first step reads B). */
+int psim_gpu_step(psim_ctx *ctx, int nsteps, const double *noise);
+/* As psim_gpu_step with noise already resident in device memory. */
+int psim_gpu_step_device_noise(psim_ctx *ctx, int nsteps, const double *d_noise);
+
+/* Forces at the current positions (AoS3, original order); no state change. */
+int psim_gpu_forces(psim_ctx *ctx, double *force_out);
+
+/* Forces from the reference's own host arrays: sorted4 = sortedParticles
+ * (x,y,z,r per sorted index), hash_index = particleHashIndex as 2n ints
+ * (hash, original index), cell_start_end = cellStartEnd as 2*S^3 ints
+ * (start, end; -1 empty).  force_out is particleForce (AoS3, original
+ * index).  n must equal the context's n_particles. */
+int psim_gpu_force_table(psim_ctx *ctx, const double *sorted4,
+                         const int *hash_index, const int *cell_start_end,
+                         double *force_out);
+
+/* Cell data of the current positions for inspection: key_by_orig[n]
+ * (cell hash per original index), cell_start[S^3+1] (slot offsets),
+ * orig_by_slot[n] (original index stored at each slot). */
+int psim_gpu_get_cells(psim_ctx *ctx, int *key_by_orig, int *cell_start,
+                       int *orig_by_slot);
+
+/* First recorded physics error since the last clear: code (0 if none) and
+ * the original indices of the particle(s) involved. */
+int psim_gpu_last_error(psim_ctx *ctx, int *code, int *i, int *j);
+int psim_gpu_clear_error(psim_ctx *ctx);
+
+int psim_gpu_sync(psim_ctx *ctx);
+int psim_gpu_set_profiling(psim_ctx *ctx, int enabled);
+int psim_gpu_kernel_times(psim_ctx *ctx, psim_kernel_times *out, int reset);
+/* The context's HIP stream (hipStream_t), for callers that order work
+ * against it. */
+void *psim_gpu_stream(psim_ctx *ctx);
+
+/* Seeded random sequential addition of n spheres with pair distance >= dmin
+ * in a periodic cube [0, box)^3 (host, O(n)).  Returns 0 or PSIM_ERR_INIT. */
+int psim_init_rsa(int n, double box, double dmin, unsigned long long seed,
+                  double *pos_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,110 @@
+"""ctypes binding of the C ABI declared in include/psim_gpu.h.
+
+The shared library is built in-tree (``python -m proteinsim2015_amd.build``)
+and loaded from this package directory.  There is no fallback: if the
+library is missing or was built for another ABI version, loading fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libpsim_gpu.so")
+ABI_VERSION = 1
+
+FORCE_CAL, FORCE_LJ, FORCE_AO = 1, 2, 4
+ERR_INIT, ERR_BOUNDS, ERR_OVERLAP, ERR_NAN = 7701, 7703, 7704, 7705
+
+
+class PsimParams(C.Structure):
+    _fields_ = [
+        ("n_particles", C.c_int), ("cells_per_dim", C.c_int),
+        ("box", C.c_double), ("cell_size", C.c_double), ("half_uv", C.c_double),
+        ("radius", C.c_double), ("mass", C.c_double), ("kT", C.c_double),
+        ("gamma", C.c_double), ("dt", C.c_double),
+        ("vel_freq", C.c_int), ("force_mask", C.c_int), ("lj_num", C.c_int),
+        ("reserved0", C.c_int),
+        ("cutoff", C.c_double), ("lj_well_depth", C.c_double),
+        ("yukawa_strength", C.c_double), ("debye_length", C.c_double),
+        ("ao_well_depth", C.c_double), ("seed", C.c_ulonglong),
+    ]
+
+
+class KernelTimes(C.Structure):
+    _fields_ = [("force_ms", C.c_double), ("force_launches", C.c_int),
+                ("build_ms", C.c_double), ("build_launches", C.c_int),
+                ("total_ms", C.c_double), ("steps", C.c_int)]
+
+
+# every exported entry point of include/psim_gpu.h
+EXPORTS = (
+    "psim_gpu_abi_version", "psim_gpu_status_string", "psim_gpu_create",
+    "psim_gpu_destroy", "psim_gpu_set_state", "psim_gpu_get_state", "psim_gpu_step",
+    "psim_gpu_step_device_noise", "psim_gpu_forces", "psim_gpu_force_table",
+    "psim_gpu_get_cells", "psim_gpu_last_error", "psim_gpu_clear_error",
+    "psim_gpu_sync", "psim_gpu_set_profiling", "psim_gpu_kernel_times",
+    "psim_gpu_stream", "psim_init_rsa",
+)
+
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+_SIGS = {
+    "psim_gpu_abi_version": ([], C.c_int),
+    "psim_gpu_status_string": ([C.c_int], C.c_char_p),
+    "psim_gpu_create": ([C.POINTER(PsimParams), C.POINTER(_vp)], C.c_int),
+    "psim_gpu_destroy": ([_vp], None),
+    "psim_gpu_set_state": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int,
+                            C.c_ulonglong], C.c_int),
+    "psim_gpu_get_state": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip,
+                            C.POINTER(C.c_ulonglong)], C.c_int),
+    "psim_gpu_step": ([_vp, C.c_int, _dp], C.c_int),
+    "psim_gpu_step_device_noise": ([_vp, C.c_int, _vp], C.c_int),
+    "psim_gpu_forces": ([_vp, _dp], C.c_int),
+    "psim_gpu_force_table": ([_vp, _dp, _ip, _ip, _dp], C.c_int),
+    "psim_gpu_get_cells": ([_vp, _ip, _ip, _ip], C.c_int),
+    "psim_gpu_last_error": ([_vp, _ip, _ip, _ip], C.c_int),
+    "psim_gpu_clear_error": ([_vp], C.c_int),
+    "psim_gpu_sync": ([_vp], C.c_int),
+    "psim_gpu_set_profiling": ([_vp, C.c_int], C.c_int),
+    "psim_gpu_kernel_times": ([_vp, C.POINTER(KernelTimes), C.c_int], C.c_int),
+    "psim_gpu_stream": ([_vp], _vp),
+    "psim_init_rsa": ([C.c_int, C.c_double, C.c_double, C.c_ulonglong, _dp], C.c_int),
+}
+
+_LIB = None
+
+
+def load() -> C.CDLL:
+    """Load libpsim_gpu.so; raises if it is missing (no CPU fallback)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m proteinsim2015_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    v = lib.psim_gpu_abi_version()
+    if v != ABI_VERSION:
+        raise RuntimeError(f"libpsim_gpu.so ABI {v} != {ABI_VERSION}")
+    _LIB = lib
+    return lib
+
+
+class PsimError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        msg = load().psim_gpu_status_string(status).decode()
+        super().__init__(f"{what}: {msg}" if what else msg)
+
+
+def check(status: int, what: str = "") -> int:
+    if status != 0:
+        raise PsimError(status, what)
+    return status

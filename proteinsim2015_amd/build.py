@@ -1,0 +1,90 @@
+"""Build the native libraries in-tree.
+
+    python -m proteinsim2015_amd.build            # product + oracle (+ reference harness)
+
+Products (git-ignored, shipped with the tree to the GPU host):
+  proteinsim2015_amd/libpsim_gpu.so        HIP kernels (gfx950) + C ABI
+  proteinsim2015_amd/plugins/*.so          PSim::IForce plugin adaptors
+Test infrastructure built alongside (never loaded by the product):
+  oracle/libpsim_oracle.so                 C restatement of the reference
+  oracle/_ref/*                            the reference's own sources, when
+                                           the reference checkout is present
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libpsim_gpu.so")
+PLUGIN_DIR = os.path.join(PKG, "plugins")
+REFERENCE = "/root/reference"
+
+ARCH = os.environ.get("PSIM_OFFLOAD_ARCH", "gfx950")
+
+# HIP device code: gfx950 only, packed-FP32 instructions disabled, no FMA
+# contraction unless written explicitly (the integrator must round like the
+# reference).
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build_product(verbose: bool = False) -> str:
+    hipcc = _hipcc()
+    objs = []
+    tmp = os.path.join(PKG, "_obj")
+    os.makedirs(tmp, exist_ok=True)
+    for src in ("psim_engine.hip",):
+        obj = os.path.join(tmp, src + ".o")
+        _run([hipcc, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c",
+              os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    for src in ("psim_host.cpp",):
+        obj = os.path.join(tmp, src + ".o")
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-I", INCLUDE, "-c",
+              os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    out = LIB + ".tmp"
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs, "-lgomp"],
+         verbose)
+    os.replace(out, LIB)
+    return LIB
+
+
+def build_oracle(verbose: bool = False) -> None:
+    sys.path.insert(0, ROOT)
+    from oracle.oracle import build_oracle as _b  # test infrastructure
+    _b(force=True)
+    if os.path.isdir(REFERENCE):
+        _run(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref"), f"REF={REFERENCE}",
+              "-j8"], verbose)
+
+
+def build_all(verbose: bool = False) -> None:
+    build_product(verbose)
+    build_oracle(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose="-v" in sys.argv)

@@ -1,0 +1,393 @@
+"""Host-side mirror of the reference's interfaces over the C ABI.
+
+Vocabulary follows the reference:
+  * ``Config``       -- PSim::config, ``key = value`` files with typed
+                        ``get_param(key, default)`` (config.cpp:27-175);
+  * ``LennardJones``, ``AOPotential``, ``Calibration`` -- the three IForce
+                        plugins' parameters with the reference's keys and
+                        defaults (LJPotential.cpp:29-62, AOPotential.cpp:33-59,
+                        Calibration.cpp:30-39);
+  * ``BrownianIntegrator`` -- brownianIntegrator's parameters
+                        (brownianIntegrator.cpp:27-99);
+  * ``SystemGeometry`` -- systemState's box/cell geometry
+                        (system.cpp:190-227);
+  * ``GpuSystem``      -- one simulated system on the GPU: the state, the
+                        force evaluation and the step loop of system::run.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import FORCE_AO, FORCE_CAL, FORCE_LJ, PsimParams, check
+
+_NUM = re.compile(r"^\s*[+-]?(\d+\.?\d*([eE][+-]?\d+)?|\.\d+([eE][+-]?\d+)?)")
+_INT = re.compile(r"^\s*[+-]?\d+")
+
+
+class Config:
+    """PSim::config: ``key = value`` lines, ``#`` comment lines, whitespace
+    trimmed (config.cpp:27-56); values converted like std::stoi/std::stod,
+    which read the leading number (so ``wellDepth = 10;`` is 10)."""
+
+    def __init__(self, source=None, **overrides):
+        self.options: dict[str, str] = {}
+        if isinstance(source, dict):
+            self.options.update({k: str(v) for k, v in source.items()})
+        elif source is not None:
+            with open(source) as f:
+                for line in f:
+                    self._parse_line(line.rstrip("\n"))
+        self.options.update({k: str(v) for k, v in overrides.items()})
+
+    def _parse_line(self, line: str) -> None:
+        if not line or line[0] == "#":
+            return
+        key, _, val = line.partition("=")
+        self.options[key.strip()] = val.strip()
+
+    def contains_key(self, key: str) -> bool:
+        return key in self.options
+
+    def get_param(self, key: str, default):
+        if key not in self.options:
+            return default
+        raw = self.options[key]
+        if isinstance(default, bool):
+            return bool(int(_INT.match(raw).group(0)))
+        if isinstance(default, int):
+            m = _INT.match(raw)
+            if not m:
+                raise ValueError(f"config: {key} = {raw!r} is not an int")
+            return int(m.group(0))
+        if isinstance(default, float):
+            m = _NUM.match(raw)
+            if not m:
+                raise ValueError(f"config: {key} = {raw!r} is not a number")
+            return float(m.group(0))
+        return raw
+
+
+@dataclass
+class Calibration:
+    """Calibration.cpp:30-39: soft shell F = -36 r^-37 inside cutOff."""
+    cutoff: float = 1.1
+    mask: int = FORCE_CAL
+
+    @classmethod
+    def from_config(cls, cfg: Config):
+        return cls(cutoff=cfg.get_param("cutOff", 1.1))
+
+
+@dataclass
+class LennardJones:
+    """LJPotential.cpp:29-62: LJ (2n-n) plus Yukawa, both scaled by kT*wellDepth."""
+    kT: float = 1.0
+    well_depth: float = 10.0
+    yukawa_strength: float = 8.0
+    lj_num: int = 18
+    cutoff: float = 2.5
+    debye_length: float = 0.5
+    mask: int = FORCE_LJ
+
+    @classmethod
+    def from_config(cls, cfg: Config):
+        return cls(kT=cfg.get_param("kT", 1.0), well_depth=cfg.get_param("wellDepth", 10.0),
+                   yukawa_strength=cfg.get_param("yukawaStrength", 8.0),
+                   lj_num=cfg.get_param("ljNum", 18), cutoff=cfg.get_param("cutOff", 2.5),
+                   debye_length=cfg.get_param("debyeLength", 0.5))
+
+
+@dataclass
+class AOPotential:
+    """AOPotential.cpp:33-59: Asakura-Oosawa attraction (polymer reservoir
+    fraction wellDepth) plus the r^-38 core; cutoff = 1 + q."""
+    kT: float = 1.0
+    well_depth: float = 0.261
+    cutoff: float = 1.1
+    mask: int = FORCE_AO
+
+    @classmethod
+    def from_config(cls, cfg: Config):
+        return cls(kT=cfg.get_param("kT", 1.0), well_depth=cfg.get_param("wellDepth", 0.261),
+                   cutoff=cfg.get_param("cutOff", 1.1))
+
+    @staticmethod
+    def well_depth_for_contact(u_contact_kT: float, q: float) -> float:
+        """phi_p giving an AO contact energy of -u kT: U_AO(1) = -phi_p (3+2q)/(2q)."""
+        return 2.0 * q * u_contact_kT / (3.0 + 2.0 * q)
+
+
+PLUGINS = {"LJ": LennardJones, "AO": AOPotential, "Cal": Calibration,
+           "LennardJones": LennardJones, "AOPotential": AOPotential,
+           "Calibration": Calibration}
+
+
+@dataclass
+class ForceSet:
+    """A sum of force modules sharing one cutoff (the reference's config has
+    a single ``cutOff`` key read by every plugin)."""
+    modules: list = field(default_factory=list)
+
+    @property
+    def mask(self) -> int:
+        m = 0
+        for mod in self.modules:
+            m |= mod.mask
+        return m
+
+    @property
+    def cutoff(self) -> float:
+        cs = {mod.cutoff for mod in self.modules}
+        if len(cs) != 1:
+            raise ValueError(f"force modules disagree on cutOff: {sorted(cs)}")
+        return cs.pop()
+
+    def fill(self, p: PsimParams) -> None:
+        p.force_mask = self.mask
+        p.cutoff = self.cutoff
+        p.lj_num, p.lj_well_depth, p.yukawa_strength, p.debye_length = 18, 10.0, 8.0, 0.5
+        p.ao_well_depth = 0.261
+        for m in self.modules:
+            if isinstance(m, LennardJones):
+                p.lj_num, p.lj_well_depth = m.lj_num, m.well_depth
+                p.yukawa_strength, p.debye_length = m.yukawa_strength, m.debye_length
+            elif isinstance(m, AOPotential):
+                p.ao_well_depth = m.well_depth
+
+
+@dataclass
+class BrownianIntegrator:
+    """brownianIntegrator.cpp:27-99 parameters (coefficients are computed in
+    the native library with the reference's formulas)."""
+    kT: float = 1.0
+    mass: float = 1.0
+    gamma: float = 0.5
+    dt: float = 0.001
+    vel_freq: int = 1000
+    seed: int = 90210
+
+    @classmethod
+    def from_config(cls, cfg: Config):
+        return cls(kT=cfg.get_param("kT", 1.0), mass=cfg.get_param("mass", 1.0),
+                   gamma=cfg.get_param("gamma", 0.5), dt=cfg.get_param("timeStep", 0.001),
+                   vel_freq=cfg.get_param("velFreq", 1000), seed=cfg.get_param("seed", 90210))
+
+
+@dataclass
+class SystemGeometry:
+    n: int
+    box: float
+    cells_per_dim: int
+    cell_size: float
+    radius: float = 0.5
+    integer: bool = True
+
+    @property
+    def ncell(self) -> int:
+        return self.cells_per_dim ** 3
+
+    @property
+    def volume_fraction(self) -> float:
+        return self.n * (4.0 / 3.0) * math.pi * self.radius ** 3 / self.box ** 3
+
+    @classmethod
+    def from_config(cls, cfg: Config):
+        """system::setSystemConstants (system.cpp:190-227): integer box and
+        cell, box = cellSize * scale."""
+        n = cfg.get_param("nParticles", 1000)
+        conc = cfg.get_param("conc", 0.01)
+        scale = cfg.get_param("scale", 4)
+        r = cfg.get_param("radius", 0.5)
+        vP = n * (4.0 / 3.0) * math.atan(1.0) * 4.0 * r * r * r
+        box = int(np.cbrt(vP / conc))
+        cell = box // scale
+        box = cell * scale
+        return cls(n=n, box=float(box), cells_per_dim=scale, cell_size=float(cell),
+                   radius=r, integer=True)
+
+    @classmethod
+    def exact(cls, n: int, phi: float, min_cell: float, radius: float = 0.5):
+        """Exact box for volume fraction phi; S = floor(L / min_cell) cells of
+        edge L / S >= min_cell (normally the cutoff)."""
+        box = (n * (4.0 / 3.0) * math.pi * radius ** 3 / phi) ** (1.0 / 3.0)
+        S = int(math.floor(box / min_cell))
+        return cls(n=n, box=box, cells_per_dim=S, cell_size=box / S, radius=radius,
+                   integer=False)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _aos3(a, n, name):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if a.size != 3 * n:
+        raise ValueError(f"{name}: expected {n}x3 values, got {a.size}")
+    return a
+
+
+def rsa_positions(n: int, box: float, seed: int, dmin: float = 1.0) -> np.ndarray:
+    """Seeded random sequential addition (native, O(N)); error 7701 if the
+    packing cannot be completed."""
+    out = np.empty(3 * n)
+    check(_lib.load().psim_init_rsa(int(n), float(box), float(dmin), int(seed), _ptr(out)),
+          "psim_init_rsa")
+    return out.reshape(n, 3)
+
+
+class GpuSystem:
+    """One system resident in HBM, advanced by the HIP kernels."""
+
+    def __init__(self, geometry: SystemGeometry, forces, integrator: BrownianIntegrator,
+                 noise_seed: int | None = None):
+        self.lib = _lib.load()
+        if not isinstance(forces, ForceSet):
+            forces = ForceSet(list(forces) if isinstance(forces, (list, tuple)) else [forces])
+        self.geometry, self.forces, self.integrator = geometry, forces, integrator
+        p = PsimParams()
+        p.n_particles = geometry.n
+        p.cells_per_dim = geometry.cells_per_dim
+        p.box = geometry.box
+        p.cell_size = geometry.cell_size
+        p.half_uv = 0.0
+        p.radius = geometry.radius
+        p.mass, p.kT, p.gamma, p.dt = (integrator.mass, integrator.kT, integrator.gamma,
+                                       integrator.dt)
+        p.vel_freq = integrator.vel_freq
+        forces.fill(p)
+        kts = {m.kT for m in forces.modules if hasattr(m, "kT")}
+        if kts and kts != {integrator.kT}:
+            raise ValueError("force modules and integrator must share kT (one config key)")
+        p.kT = integrator.kT
+        p.seed = integrator.seed if noise_seed is None else noise_seed
+        self.params = p
+        self._ctx = C.c_void_p()
+        check(self.lib.psim_gpu_create(C.byref(p), C.byref(self._ctx)), "psim_gpu_create")
+        self.n = geometry.n
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if self._ctx:
+            self.lib.psim_gpu_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state --------------------------------------------------------------
+    def set_state(self, pos, pos0=None, vel=None, frc=None, memx=None, time=0.0,
+                  vel_counter=0, step_index=0):
+        n = self.n
+        arrs = [_aos3(a, n, k) for a, k in ((pos, "pos"), (pos0, "pos0"), (vel, "vel"),
+                                            (frc, "frc"), (memx, "memx"))]
+        check(self.lib.psim_gpu_set_state(self._ctx, _ptr(arrs[0]), _ptr(arrs[1]),
+                                          _ptr(arrs[2]), _ptr(arrs[3]), None, _ptr(arrs[4]),
+                                          float(time), int(vel_counter), int(step_index)),
+              "psim_gpu_set_state")
+
+    def get_state(self) -> dict:
+        n = self.n
+        out = {k: np.empty(3 * n) for k in ("pos", "pos0", "vel", "frc", "memx")}
+        t = C.c_double()
+        vc = C.c_int()
+        si = C.c_ulonglong()
+        check(self.lib.psim_gpu_get_state(self._ctx, _ptr(out["pos"]), _ptr(out["pos0"]),
+                                          _ptr(out["vel"]), _ptr(out["frc"]), None,
+                                          _ptr(out["memx"]), C.byref(t), C.byref(vc),
+                                          C.byref(si)), "psim_gpu_get_state")
+        st = {k: v.reshape(n, 3) for k, v in out.items()}
+        st.update(time=t.value, vel_counter=vc.value, step_index=si.value)
+        return st
+
+    # -- hot path -------------------------------------------------------------
+    def step(self, nsteps: int = 1, noise=None) -> int:
+        """Advance nsteps; noise None -> in-kernel Philox, else [nsteps][6][n].
+        Returns the status (0 or 7703/7704/7705)."""
+        nz = None
+        if noise is not None:
+            nz = np.ascontiguousarray(noise, dtype=np.float64).reshape(-1)
+            if nz.size != nsteps * 6 * self.n:
+                raise ValueError("noise must be [nsteps][6][n]")
+        st = self.lib.psim_gpu_step(self._ctx, int(nsteps), _ptr(nz))
+        if st < 0:
+            check(st, "psim_gpu_step")
+        return st
+
+    def step_device_noise(self, nsteps: int, device_ptr: int) -> None:
+        check(self.lib.psim_gpu_step_device_noise(self._ctx, int(nsteps),
+                                                  C.c_void_p(device_ptr)),
+              "psim_gpu_step_device_noise")
+
+    def forces_now(self):
+        """(status, forces AoS3 by original index) at the current positions."""
+        out = np.empty(3 * self.n)
+        st = self.lib.psim_gpu_forces(self._ctx, _ptr(out))
+        if st < 0:
+            check(st, "psim_gpu_forces")
+        return st, out.reshape(self.n, 3)
+
+    def force_table(self, sorted4, hash_index, cell_start_end):
+        """IForce::getAcceleration over the reference's own arrays."""
+        s4 = np.ascontiguousarray(sorted4, dtype=np.float64).reshape(-1)
+        hi = np.ascontiguousarray(hash_index, dtype=np.int32).reshape(-1)
+        cse = np.ascontiguousarray(cell_start_end, dtype=np.int32).reshape(-1)
+        out = np.empty(3 * self.n)
+        st = self.lib.psim_gpu_force_table(self._ctx, _ptr(s4),
+                                           hi.ctypes.data_as(C.POINTER(C.c_int)),
+                                           cse.ctypes.data_as(C.POINTER(C.c_int)), _ptr(out))
+        if st < 0:
+            check(st, "psim_gpu_force_table")
+        return st, out.reshape(self.n, 3)
+
+    def cells(self):
+        n, nc = self.n, self.geometry.ncell
+        key = np.empty(n, np.int32)
+        cs = np.empty(nc + 1, np.int32)
+        ob = np.empty(n, np.int32)
+        ip = C.POINTER(C.c_int)
+        check(self.lib.psim_gpu_get_cells(self._ctx, key.ctypes.data_as(ip),
+                                          cs.ctypes.data_as(ip), ob.ctypes.data_as(ip)),
+              "psim_gpu_get_cells")
+        return key, cs, ob
+
+    def last_error(self):
+        code, i, j = C.c_int(), C.c_int(), C.c_int()
+        check(self.lib.psim_gpu_last_error(self._ctx, C.byref(code), C.byref(i), C.byref(j)),
+              "psim_gpu_last_error")
+        return code.value, i.value, j.value
+
+    def clear_error(self):
+        check(self.lib.psim_gpu_clear_error(self._ctx), "psim_gpu_clear_error")
+
+    def sync(self):
+        check(self.lib.psim_gpu_sync(self._ctx), "psim_gpu_sync")
+
+    def set_profiling(self, on: bool):
+        check(self.lib.psim_gpu_set_profiling(self._ctx, int(on)), "psim_gpu_set_profiling")
+
+    def kernel_times(self, reset=False) -> dict:
+        kt = _lib.KernelTimes()
+        check(self.lib.psim_gpu_kernel_times(self._ctx, C.byref(kt), int(reset)),
+              "psim_gpu_kernel_times")
+        return {f: getattr(kt, f) for f, _ in kt._fields_}
+
+    @property
+    def stream(self) -> int:
+        return self.lib.psim_gpu_stream(self._ctx) or 0
