@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--path", default="auto", choices=["auto", "sparse", "dense"])
     return ap.parse_args()
 
 
@@ -123,7 +124,7 @@ def main():
                          cutoff=CUTOFF, debye_length=DEBYE)
     integ = ps.BrownianIntegrator(kT=1.0, mass=1.0, gamma=1.0, dt=1e-3, vel_freq=1000,
                                   seed=90210)
-    sysg = ps.GpuSystem(geom, lj, integ)
+    sysg = ps.GpuSystem(geom, lj, integ, kernel_path=args.path)
     sysg.set_state(pos, vel=vel)
     setup_s = time.perf_counter() - t_setup
     print(f"bench: setup {setup_s:.1f}s (RSA + upload), warmup {args.warmup}", file=sys.stderr,

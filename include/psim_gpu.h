@@ -80,7 +80,8 @@ typedef struct psim_params {
     int vel_freq;           /* velFreq                                      */
     int force_mask;         /* PSIM_FORCE_* bits                            */
     int lj_num;             /* ljNum                                        */
-    int reserved0;
+    int kernel_path;        /* 0 auto (by density), 1 thread per particle,
+                               2 cell per workgroup (needs S >= 4)          */
     double cutoff;          /* cutOff (shared by all modules)               */
     double lj_well_depth;   /* wellDepth of the LJ module (epsilon)         */
     double yukawa_strength; /* yukawaStrength                               */
@@ -94,9 +95,12 @@ typedef struct psim_ctx psim_ctx;
 /* per-kernel device time accumulated since the last reset (HIP events on
  * the context's stream), in milliseconds, with launch counts */
 typedef struct psim_kernel_times {
-    double force_ms;   int force_launches;
-    double build_ms;   int build_launches;   /* scan + scatter + reorder */
-    double total_ms;   int steps;
+    double force_ms;     int force_launches;     /* pair kernel (fused G&B on
+                                                    the thread-per-particle path) */
+    double integrate_ms; int integrate_launches; /* separate G&B kernel (cell-
+                                                    per-workgroup path) */
+    double build_ms;     int build_launches;     /* scan + scatter + reorder */
+    double total_ms;     int steps;
 } psim_kernel_times;
 
 int psim_gpu_abi_version(void);

@@ -24,7 +24,7 @@ class PsimParams(C.Structure):
         ("radius", C.c_double), ("mass", C.c_double), ("kT", C.c_double),
         ("gamma", C.c_double), ("dt", C.c_double),
         ("vel_freq", C.c_int), ("force_mask", C.c_int), ("lj_num", C.c_int),
-        ("reserved0", C.c_int),
+        ("kernel_path", C.c_int),
         ("cutoff", C.c_double), ("lj_well_depth", C.c_double),
         ("yukawa_strength", C.c_double), ("debye_length", C.c_double),
         ("ao_well_depth", C.c_double), ("seed", C.c_ulonglong),
@@ -33,6 +33,7 @@ class PsimParams(C.Structure):
 
 class KernelTimes(C.Structure):
     _fields_ = [("force_ms", C.c_double), ("force_launches", C.c_int),
+                ("integrate_ms", C.c_double), ("integrate_launches", C.c_int),
                 ("build_ms", C.c_double), ("build_launches", C.c_int),
                 ("total_ms", C.c_double), ("steps", C.c_int)]
 

@@ -1,0 +1,202 @@
+// psim_dense.hpp -- cell-per-workgroup pair kernel for dense systems.
+//
+// One workgroup (kDenseThreads lanes) owns one home cell.  It stages the 27
+// neighbour cells' positions in LDS, in the reference's stencil order
+// (x outer, z inner; LJPotential.cpp:154-177), with the periodic image shift
+// of each neighbour cell applied once while staging.  The n_i home particles
+// are spread over G = kDenseThreads / n_i lane groups; lane (i, g) culls
+// candidates g, g+G, ... of the staged list, keeps survivors in a per-lane
+// LDS hit list and drains it (pair forces) whenever any lane of its wave
+// fills it.  Group partials are summed through LDS, then lanes 0..n_i-1 run
+// the integrator epilogue for the home particles.
+//
+// Neighbour sets are bit-exact with the reference: fast r^2 on the shifted
+// image with a guard band, and util::pbcDist's exact arithmetic on the raw
+// positions (utilities.cpp:79-104) for any survivor inside the band.
+// Requires S >= 4 (the shifted image is then the minimum image).
+#pragma once
+
+#include "psim_device.hpp"
+
+namespace psim {
+
+constexpr int kDenseThreads = 128;   // two wave64s per home cell
+constexpr int kStageCap = 512;       // staged candidates per chunk
+constexpr int kDenseHitCap = 32;     // per-lane hit-list entries
+
+struct DenseShared {
+    double sx[kStageCap], sy[kStageCap], sz[kStageCap];
+    int sslot[kStageCap];
+    unsigned short hits[kDenseHitCap * kDenseThreads];
+    double red[3][kDenseThreads];
+    int coff[28];
+    int cbase[27];
+    double shx[27], shy[27], shz[27];
+};
+
+// Map a workgroup index to a cell so that each XCD (workgroups are handed
+// out round-robin over 8 XCDs) walks one contiguous slab of cells and its
+// private L2 keeps the slab's positions.
+__device__ inline int xcd_cell(int b, int per_xcd)
+{
+    return (b & 7) * per_xcd + (b >> 3);
+}
+
+template <int MASK>
+__device__ void dense_drain(const Geo &g, const Pot &p, const DenseShared &sh,
+                            const double *__restrict__ pos, double xi, double yi,
+                            double zi, double size, double ov2_hi, int self, int lane,
+                            int nh, double &fx,
+                            double &fy, double &fz, DevErr *err, int id_i,
+                            const int *__restrict__ id)
+{
+    const int N = g.n;
+    for (int k = 0; k < nh; ++k) {
+        const int j = sh.hits[k * kDenseThreads + lane];
+        if (j == self) continue;
+        const double dx = sh.sx[j] - xi, dy = sh.sy[j] - yi, dz = sh.sz[j] - zi;
+        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        if (r2 >= p.rc2_lo || r2 < ov2_hi) {
+            const int sj = sh.sslot[j];
+            const double r2x = pbc_dist_ref(xi, yi, zi, pos[sj], pos[N + sj], pos[2 * N + sj],
+                                            g.L, g.half_pbc);
+            if (r2 >= p.rc2_lo && !(r2x < p.rc2)) continue;
+            if (sqrt(r2x) < p.ov_factor * size) report_error(err, 7704, id_i, id[sj]);
+        }
+        const double rinv = rsqrt_nr(r2);
+        const double r = r2 * rinv;
+        bool nan_ao = false;
+        const double f = pair_fnet<MASK>(p, r2, rinv, r, size, &nan_ao);
+        if (nan_ao) report_error(err, 7705, id_i, id[sh.sslot[j]]);
+        const double fr = f * rinv;
+        fx = fma(fr, dx, fx);
+        fy = fma(fr, dy, fy);
+        fz = fma(fr, dz, fz);
+    }
+}
+
+// Forces on the home particles of cell c.  On return, lanes 0..ni-1 hold the
+// total force on home particle (i0 + lane) for each i-chunk via the callback
+// epi(i, fx, fy, fz) (called by those lanes only).
+template <int MASK, class Epi>
+__device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
+                                  const double *__restrict__ pos,
+                                  const int *__restrict__ cstart,
+                                  const int *__restrict__ id, DevErr *err, int c,
+                                  DenseShared &sh, Epi epi)
+{
+    const int N = g.n, S = g.S;
+    const int lane = threadIdx.x;
+    const int cx = c % S, cy = (c / S) % S, cz = c / g.S2;
+    const int home0 = cstart[c];
+    const int ni = cstart[c + 1] - home0;
+    if (ni == 0) return;   // uniform over the workgroup
+    // ---- neighbour-cell metadata (lanes 0..26, stencil order)
+    int cnt = 0;
+    if (lane < 27) {
+        const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
+        int rx = cx + ox, ry = cy + oy, rz = cz + oz;
+        double sxv = 0.0, syv = 0.0, szv = 0.0;
+        if (rx < 0) { rx += S; sxv = -g.L; } else if (rx >= S) { rx -= S; sxv = g.L; }
+        if (ry < 0) { ry += S; syv = -g.L; } else if (ry >= S) { ry -= S; syv = g.L; }
+        if (rz < 0) { rz += S; szv = -g.L; } else if (rz >= S) { rz -= S; szv = g.L; }
+        const int h = rx + S * ry + g.S2 * rz;
+        const int b = cstart[h];
+        cnt = cstart[h + 1] - b;
+        sh.cbase[lane] = b;
+        sh.shx[lane] = sxv;
+        sh.shy[lane] = syv;
+        sh.shz[lane] = szv;
+    }
+    if (lane < 64) {
+        int incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 32; d <<= 1) {
+            int y = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += y;
+        }
+        if (lane < 27) sh.coff[lane] = incl - cnt;
+        if (lane == 26) sh.coff[27] = incl;
+    }
+    __syncthreads();
+    const int M = sh.coff[27];
+    const int hb = sh.coff[13];   // staged index of the first home particle
+    const double size = 2.0 * radius;
+    const double ov = p.ov_factor * size;
+    const double ov2_hi = ov * ov * (1.0 + p.ov_band);
+
+    for (int i0 = 0; i0 < ni; i0 += kDenseThreads) {
+        const int nic = min(kDenseThreads, ni - i0);
+        const int G = kDenseThreads / nic;
+        const int gi = lane % nic, gg = lane / nic;
+        const bool act = gg < G;
+        const int si = home0 + i0 + gi;       // global slot of this lane's i
+        const int hi = hb + i0 + gi;          // staged index of i
+        const double xi = pos[si], yi = pos[N + si], zi = pos[2 * N + si];
+        const int id_i = id[si];
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int s0 = 0; s0 < M; s0 += kStageCap) {
+            const int mc = min(kStageCap, M - s0);
+            __syncthreads();   // the previous chunk is no longer read
+            for (int k = lane; k < mc; k += kDenseThreads) {
+                const int t = s0 + k;
+                int lo = 0, hi2 = 26;   // last q with coff[q] <= t
+                while (lo < hi2) {
+                    const int mid = (lo + hi2 + 1) >> 1;
+                    if (sh.coff[mid] <= t) lo = mid; else hi2 = mid - 1;
+                }
+                const int slot = sh.cbase[lo] + (t - sh.coff[lo]);
+                sh.sx[k] = pos[slot] + sh.shx[lo];
+                sh.sy[k] = pos[N + slot] + sh.shy[lo];
+                sh.sz[k] = pos[2 * N + slot] + sh.shz[lo];
+                sh.sslot[k] = slot;
+            }
+            __syncthreads();
+            if (act) {
+                // self is accepted by the cull (r^2 = 0) and skipped in the drain
+                int nh = 0;
+                int j = gg;
+                for (; j + 3 * G < mc; j += 4 * G) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int jj = j + u * G;
+                        const double dx = sh.sx[jj] - xi, dy = sh.sy[jj] - yi, dz = sh.sz[jj] - zi;
+                        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                        sh.hits[nh * kDenseThreads + lane] = (unsigned short)jj;
+                        nh += (r2 < p.rc2_hi) ? 1 : 0;
+                    }
+                    if (__any(nh > kDenseHitCap - 4)) {
+                        dense_drain<MASK>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, hi - s0,
+                                          lane, nh, fx, fy, fz, err, id_i, id);
+                        nh = 0;
+                    }
+                }
+                for (; j < mc; j += G) {
+                    const double dx = sh.sx[j] - xi, dy = sh.sy[j] - yi, dz = sh.sz[j] - zi;
+                    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    sh.hits[nh * kDenseThreads + lane] = (unsigned short)j;
+                    nh += (r2 < p.rc2_hi) ? 1 : 0;
+                }
+                dense_drain<MASK>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, hi - s0, lane, nh,
+                                  fx, fy, fz, err, id_i, id);
+            }
+        }
+        // ---- sum the G group partials of each home particle
+        sh.red[0][lane] = fx;
+        sh.red[1][lane] = fy;
+        sh.red[2][lane] = fz;
+        __syncthreads();
+        if (lane < nic) {
+            double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+            for (int q = 0; q < G; ++q) {
+                Fx += sh.red[0][lane + q * nic];
+                Fy += sh.red[1][lane + q * nic];
+                Fz += sh.red[2][lane + q * nic];
+            }
+            epi(i0 + lane, Fx, Fy, Fz);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace psim

@@ -109,7 +109,7 @@ __device__ inline double min_image_uv(double d, double L, double half_uv)
 }
 
 template <int E>
-__device__ inline double pow_const(double b)
+__device__ __forceinline__ double pow_const(double b)
 {
     double a = 1.0;
     int e = E;
@@ -122,7 +122,7 @@ __device__ inline double pow_const(double b)
     return a;
 }
 
-__device__ inline double pow_int(double b, int e)
+__device__ __forceinline__ double pow_int(double b, int e)
 {
     double a = 1.0;
     while (e) {
@@ -133,12 +133,65 @@ __device__ inline double pow_int(double b, int e)
     return a;
 }
 
+// 1/sqrt(x) for normal x: hardware estimate (v_rsq_f64) refined by two
+// Newton steps y <- y + y (1/2 - (x/2) y^2); relative error ~1 ulp.
+__device__ __forceinline__ double rsqrt_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    return y;
+}
+
+// e^{-a} for 0 <= a < 700: a = k ln2 + f with |f| <= ln2/2 (two-part ln2),
+// e^{-f} by its Taylor series to degree 13 (truncation < 3e-17 relative),
+// then scaled by 2^{-k}.
+__device__ __forceinline__ double exp_neg(double a)
+{
+    const double log2e = 1.4426950408889634;
+    const double ln2_hi = 6.93147180369123816490e-01;
+    const double ln2_lo = 1.90821492927058770002e-10;
+    const double k = rint(a * log2e);
+    double f = fma(-k, ln2_hi, a);
+    f = fma(-k, ln2_lo, f);
+    const double x = -f;
+    double p = 1.0 / 6227020800.0;            // 1/13!
+    p = fma(p, x, 1.0 / 479001600.0);         // 1/12!
+    p = fma(p, x, 1.0 / 39916800.0);
+    p = fma(p, x, 1.0 / 3628800.0);
+    p = fma(p, x, 1.0 / 362880.0);
+    p = fma(p, x, 1.0 / 40320.0);
+    p = fma(p, x, 1.0 / 5040.0);
+    p = fma(p, x, 1.0 / 720.0);
+    p = fma(p, x, 1.0 / 120.0);
+    p = fma(p, x, 1.0 / 24.0);
+    p = fma(p, x, 1.0 / 6.0);
+    p = fma(p, x, 0.5);
+    p = fma(p, x, 1.0);
+    p = fma(p, x, 1.0);
+    return ldexp(p, -(int)k);
+}
+
+// (size/r)^n for the LJ module, n = ljNum; the common exponents unrolled.
+__device__ __forceinline__ double lj_power(double sr, int n)
+{
+    const double x = sr * sr;
+    switch (n) {
+    case 6: return x * x * x;
+    case 12: { double x2 = x * x; double x4 = x2 * x2; return x4 * x2; }
+    case 18: { double x2 = x * x; double x4 = x2 * x2; double x8 = x4 * x4; return x8 * x; }
+    case 24: { double x2 = x * x; double x4 = x2 * x2; double x8 = x4 * x4; return x8 * x4; }
+    default: return pow_int(sr, n);
+    }
+}
+
 // Pair force magnitude (positive = attractive) summed over the enabled
 // modules.  r2, rinv = 1/sqrt(r2), r = r2*rinv; size = r_i + r_j.
 // Sets *nan_ao when the AO term is NaN (AOPotential.cpp:105-109).
 template <int MASK>
-__device__ inline double pair_fnet(const Pot &p, double r2, double rinv,
-                                   double r, double size, bool *nan_ao)
+__device__ __forceinline__ double pair_fnet(const Pot &p, double r2, double rinv,
+                                            double r, double size, bool *nan_ao)
 {
     double f = 0.0;
     if (MASK & F_CAL) {
@@ -147,23 +200,53 @@ __device__ inline double pair_fnet(const Pot &p, double r2, double rinv,
     }
     if (MASK & F_LJ) {
         // LJPotential.cpp:97-109
-        double lj = pow_int(size * rinv, p.lj_num);
-        double attract = fma(2.0, lj, -1.0) * (p.lj_four_n * rinv * lj);
+        const double lj = lj_power(size * rinv, p.lj_num);
+        const double attract = fma(2.0, lj, -1.0) * (p.lj_four_n * rinv * lj);
         double repel = 0.0;
         if (p.yuk_a != 0.0) {
-            double ye = exp(-(r * p.debye_inv));
+            const double ye = exp_neg(r * p.debye_inv);
             repel = ye * (rinv * rinv * (p.debye_len + r) * p.yuk_a);
         }
         f += p.lj_neg_kt_eps * (attract + repel);
     }
     if (MASK & F_AO) {
         // AOPotential.cpp:95-102: -(36 r^-38 + C1/r + C2 r)
-        double r36 = pow_const<36>(rinv);
-        double fa = -(fma(36.0 * r36, rinv * rinv, fma(p.ao_c1, rinv, p.ao_c2 * r)));
+        const double r2inv = rinv * rinv;
+        const double r36 = pow_const<18>(r2inv);
+        const double fa = -(fma(36.0 * r36, r2inv, fma(p.ao_c1, rinv, p.ao_c2 * r)));
         if (isnan(fa)) *nan_ao = true;
         f += fa;
     }
     return f;
+}
+
+// Per-cell rank through the count array.  Adjacent active lanes with equal
+// keys form a run; the run's first lane reserves the run length with one
+// atomicAdd and the lanes take consecutive ranks.  (The particles of a cell
+// sit in consecutive slots, so per-lane atomics would serialise on a few
+// addresses; a key split over several runs simply reserves several ranges.)
+__device__ __forceinline__ int wave_aggregated_rank(int *count, int key)
+{
+    const int lane = (int)__lane_id();
+    const unsigned long long active = __ballot(1);
+    const int prev_key = __shfl_up(key, 1);
+    const bool prev_active = lane > 0 && ((active >> (lane - 1)) & 1ull);
+    const bool start = !prev_active || prev_key != key;
+    const unsigned long long starts = __ballot(start);
+    // this lane's run leader: highest start at or below the lane
+    const unsigned long long le = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const int leader = 63 - __clzll((long long)(starts & le));
+    // run end: lowest start above the lane, else past the last active lane
+    const unsigned long long above = starts & ~le;
+    const int end = above ? (__ffsll((long long)above) - 1) : 64;
+    int base = 0;
+    if (start) {
+        const unsigned long long span = (end == 64 ? ~0ull : ((1ull << end) - 1ull)) &
+                                        ~((1ull << lane) - 1ull);
+        base = atomicAdd(&count[key], __popcll(active & span));
+    }
+    base = __shfl(base, leader);
+    return base + __popcll(active & le & ~((1ull << leader) - 1ull)) - 1;
 }
 
 // ---------------------------------------------------------------------------

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "psim_gpu.h"
+#include "psim_dense.hpp"
 #include "psim_pairs.hpp"
 
 using namespace psim;
@@ -37,6 +38,8 @@ using namespace psim;
 namespace {
 
 constexpr int kScanThreads = 1024;
+// mean particles per cell above which the cell-per-workgroup kernel is used
+constexpr double kDenseMinOccupancy = 2.0;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
@@ -56,27 +59,13 @@ struct StepArgs {
     int first, do_vel;
 };
 
-template <int MASK>
-__global__ __launch_bounds__(kBlock) void k_step(Geo g, Pot p, GB gb, double radius,
-                                                 StepArgs a)
+// G&B epilogue for the particle in slot s with total force F
+// (brownianIntegrator.cpp:191-356, particle.cpp:61-99); writes the next state
+// at slot s, the new cell key and an atomic per-cell rank.
+__device__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int s,
+                              double fx, double fy, double fz)
 {
-    __shared__ unsigned hits[kHitCap * kBlock];
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= g.n) return;
     const int N = g.n;
-    const double xi = a.pos[s], yi = a.pos[N + s], zi = a.pos[2 * N + s];
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    SoAPos src{a.pos, N, radius};
-    StartTab tab{a.cstart};
-    Orig orig{a.id, 1, 0};
-    if (g.exact_mi)
-        gather_force<MASK, true>(g, p, src, tab, s, xi, yi, zi, radius,
-                                 hits + threadIdx.x, fx, fy, fz, a.err, orig);
-    else
-        gather_force<MASK, false>(g, p, src, tab, s, xi, yi, zi, radius,
-                                  hits + threadIdx.x, fx, fy, fz, a.err, orig);
-
-    // ---- G&B epilogue (brownianIntegrator.cpp:191-356, particle.cpp:61-99)
     const int o = a.id[s];
     double z[6];
     if (a.noise) {
@@ -85,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_step(Geo g, Pot p, GB gb, double rad
     } else {
         philox_normals6(a.seed, a.step, (uint32_t)o, z);
     }
-    const double xc[3] = {xi, yi, zi};
+    const double xc[3] = {a.pos[s], a.pos[N + s], a.pos[2 * N + s]};
     const double F[3] = {fx, fy, fz};
     double x0c[3], xn[3], M[3];
 #pragma unroll
@@ -152,7 +141,65 @@ __global__ __launch_bounds__(kBlock) void k_step(Geo g, Pot p, GB gb, double rad
     int k = cell_1d(xw[0], g.cell, g.S) + g.S * cell_1d(xw[1], g.cell, g.S)
           + g.S2 * cell_1d(xw[2], g.cell, g.S);
     a.key[s] = k;
-    a.arank[s] = atomicAdd(&a.count[k], 1);
+    a.arank[s] = wave_aggregated_rank(a.count, k);
+}
+
+// Thread-per-particle step (sparse systems and S < 4).
+template <int MASK>
+__global__ __launch_bounds__(kBlock) void k_step(Geo g, Pot p, GB gb, double radius,
+                                                 StepArgs a)
+{
+    __shared__ unsigned hits[kHitCap * kBlock];
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= g.n) return;
+    const int N = g.n;
+    const double xi = a.pos[s], yi = a.pos[N + s], zi = a.pos[2 * N + s];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    SoAPos src{a.pos, N, radius};
+    StartTab tab{a.cstart};
+    Orig orig{a.id, 1, 0};
+    if (g.exact_mi)
+        gather_force<MASK, true>(g, p, src, tab, s, xi, yi, zi, radius,
+                                 hits + threadIdx.x, fx, fy, fz, a.err, orig);
+    else
+        gather_force<MASK, false>(g, p, src, tab, s, xi, yi, zi, radius,
+                                  hits + threadIdx.x, fx, fy, fz, a.err, orig);
+    integrate_one(g, gb, a, s, fx, fy, fz);
+}
+
+// Cell-per-workgroup forces for the step (dense systems, S >= 4): the total
+// force on each particle goes to nfrc[slot], where k_integrate picks it up.
+// (Fusing the G&B epilogue here would run it on n_i of the workgroup's lanes,
+// latency-bound; the separate kernel runs it at full occupancy.)
+template <int MASK>
+__global__ __launch_bounds__(kDenseThreads) void k_step_dense(Geo g, Pot p, double radius,
+                                                              const double *pos,
+                                                              const int *id,
+                                                              const int *cstart,
+                                                              double *nfrc, DevErr *err,
+                                                              int per_xcd)
+{
+    __shared__ DenseShared sh;
+    const int c = xcd_cell(blockIdx.x, per_xcd);
+    if (c >= g.ncell) return;
+    const int home0 = cstart[c];
+    const int N = g.n;
+    dense_cell_forces<MASK>(g, p, radius, pos, cstart, id, err, c, sh,
+                            [=](int i, double Fx, double Fy, double Fz) {
+                                const int s = home0 + i;
+                                nfrc[s] = Fx;
+                                nfrc[N + s] = Fy;
+                                nfrc[2 * N + s] = Fz;
+                            });
+}
+
+// G&B step for every slot with the force already in nfrc (dense path).
+__global__ __launch_bounds__(256) void k_integrate(Geo g, GB gb, StepArgs a)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= g.n) return;
+    const int N = g.n;
+    integrate_one(g, gb, a, s, a.nfrc[s], a.nfrc[N + s], a.nfrc[2 * N + s]);
 }
 
 // Forces only at the current state, written by original index ([3][N]).
@@ -181,6 +228,28 @@ __global__ __launch_bounds__(kBlock) void k_forces(Geo g, Pot p, double radius,
     fout[o] = fx;
     fout[N + o] = fy;
     fout[2 * N + o] = fz;
+}
+
+template <int MASK>
+__global__ __launch_bounds__(kDenseThreads) void k_forces_dense(Geo g, Pot p, double radius,
+                                                                const double *pos,
+                                                                const int *id,
+                                                                const int *cstart,
+                                                                double *fout, DevErr *err,
+                                                                int per_xcd)
+{
+    __shared__ DenseShared sh;
+    const int c = xcd_cell(blockIdx.x, per_xcd);
+    if (c >= g.ncell) return;
+    const int home0 = cstart[c];
+    const int N = g.n;
+    dense_cell_forces<MASK>(g, p, radius, pos, cstart, id, err, c, sh,
+                            [=](int i, double Fx, double Fy, double Fz) {
+                                const int o = id[home0 + i];
+                                fout[o] = Fx;
+                                fout[N + o] = Fy;
+                                fout[2 * N + o] = Fz;
+                            });
 }
 
 // Forces from the reference's host arrays (IForce::getAcceleration over the
@@ -221,7 +290,7 @@ __global__ void k_hash_count(Geo g, const double *pos, int *key, int *arank, int
     int k = cell_1d(pos[s], g.cell, g.S) + g.S * cell_1d(pos[N + s], g.cell, g.S)
           + g.S2 * cell_1d(pos[2 * N + s], g.cell, g.S);
     key[s] = k;
-    arank[s] = atomicAdd(&count[k], 1);
+    arank[s] = wave_aggregated_rank(count, k);
 }
 
 // Exclusive scan over a 1024-thread block of one value per thread.
@@ -348,6 +417,7 @@ struct psim_ctx {
     Pot pot;
     GB gb;
     int n = 0, ncell = 0, nscan = 0;
+    bool dense = false;   // cell-per-workgroup pair kernel (else thread per particle)
     hipStream_t stream = nullptr;
     double *pos = nullptr, *pos0 = nullptr, *frc = nullptr, *mem = nullptr;
     double *npos = nullptr, *npos0 = nullptr, *nfrc = nullptr, *nmem = nullptr;
@@ -367,7 +437,7 @@ struct psim_ctx {
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
-    struct Mark { size_t e0, e1, e2; };
+    struct Mark { size_t e0, e1, e1b, e2; };
     std::vector<Mark> marks;
     psim_kernel_times times{};
 };
@@ -531,14 +601,17 @@ int collect_times(psim_ctx *c)
     if (c->marks.empty()) return 0;
     CK(hipStreamSynchronize(c->stream));
     for (const auto &mk : c->marks) {
-        float f = 0.f, b = 0.f;
+        float f = 0.f, it = 0.f, b = 0.f;
         CK(hipEventElapsedTime(&f, c->ev_pool[mk.e0], c->ev_pool[mk.e1]));
-        CK(hipEventElapsedTime(&b, c->ev_pool[mk.e1], c->ev_pool[mk.e2]));
+        CK(hipEventElapsedTime(&it, c->ev_pool[mk.e1], c->ev_pool[mk.e1b]));
+        CK(hipEventElapsedTime(&b, c->ev_pool[mk.e1b], c->ev_pool[mk.e2]));
         c->times.force_ms += f;
         c->times.force_launches += 1;
+        c->times.integrate_ms += it;
+        c->times.integrate_launches += c->dense ? 1 : 0;
         c->times.build_ms += b;
         c->times.build_launches += 1;
-        c->times.total_ms += f + b;
+        c->times.total_ms += f + it + b;
         c->times.steps += 1;
     }
     c->marks.clear();
@@ -565,8 +638,14 @@ int launch_build(psim_ctx *c)
 template <int MASK>
 void launch_step_mask(psim_ctx *c, const StepArgs &a)
 {
-    k_step<MASK><<<blocks_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->geo, c->pot, c->gb,
-                                                                    c->prm.radius, a);
+    if (c->dense) {
+        const int per = (c->ncell + 7) / 8;
+        k_step_dense<MASK><<<8 * per, kDenseThreads, 0, c->stream>>>(
+            c->geo, c->pot, c->prm.radius, a.pos, a.id, a.cstart, a.nfrc, a.err, per);
+    } else {
+        k_step<MASK><<<blocks_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->geo, c->pot, c->gb,
+                                                                        c->prm.radius, a);
+    }
 }
 
 int launch_step(psim_ctx *c, const StepArgs &a)
@@ -588,8 +667,14 @@ int launch_step(psim_ctx *c, const StepArgs &a)
 template <int MASK>
 void launch_forces_mask(psim_ctx *c)
 {
-    k_forces<MASK><<<blocks_for(c->n, kBlock), kBlock, 0, c->stream>>>(
-        c->geo, c->pot, c->prm.radius, c->pos, c->id, c->cstart, c->fout, c->err);
+    if (c->dense) {
+        const int per = (c->ncell + 7) / 8;
+        k_forces_dense<MASK><<<8 * per, kDenseThreads, 0, c->stream>>>(
+            c->geo, c->pot, c->prm.radius, c->pos, c->id, c->cstart, c->fout, c->err, per);
+    } else {
+        k_forces<MASK><<<blocks_for(c->n, kBlock), kBlock, 0, c->stream>>>(
+            c->geo, c->pot, c->prm.radius, c->pos, c->id, c->cstart, c->fout, c->err);
+    }
 }
 
 template <int MASK>
@@ -629,6 +714,14 @@ int run_steps(psim_ctx *c, int nsteps, const double *d_noise)
         if (rc) return rc;
         if (c->profiling) {
             hipEvent_t e = take_event(c, &mk.e1);
+            if (e) CK(hipEventRecord(e, c->stream));
+        }
+        if (c->dense) {
+            k_integrate<<<blocks_for(n, 256), 256, 0, c->stream>>>(c->geo, c->gb, a);
+            CK(hipGetLastError());
+        }
+        if (c->profiling) {
+            hipEvent_t e = take_event(c, &mk.e1b);
             if (e) CK(hipEventRecord(e, c->stream));
         }
         rc = launch_build(c);
@@ -693,6 +786,7 @@ int psim_gpu_create(const psim_params *params, psim_ctx **out)
     // the 27-cell stencil repeats cells for S < 3 (double counting in the
     // reference, LJPotential.cpp:154-169)
     if (P.cells_per_dim < 3 || P.cells_per_dim > 1290) return PSIM_E_UNSUPPORTED;
+    if (P.kernel_path < 0 || P.kernel_path > 2) return PSIM_E_ARG;
     if (!(P.box > 0) || !(P.cell_size > 0) || !(P.cutoff > 0) || !(P.dt > 0) ||
         !(P.mass > 0) || P.force_mask <= 0 || P.force_mask > 7)
         return PSIM_E_ARG;
@@ -704,6 +798,14 @@ int psim_gpu_create(const psim_params *params, psim_ctx **out)
     c->n = P.n_particles;
     c->ncell = c->geo.ncell;
     c->nscan = (c->ncell + kScanTile - 1) / kScanTile;
+    // kernel path: the cell-per-workgroup kernel needs S >= 4 (shifted image
+    // = minimum image) and pays off once cells hold a few particles
+    if (P.kernel_path == 1)
+        c->dense = false;
+    else if (P.kernel_path == 2)
+        c->dense = c->geo.S >= 4;
+    else
+        c->dense = c->geo.S >= 4 && (double)c->n / c->ncell >= kDenseMinOccupancy;
     const size_t n3 = 3 * (size_t)c->n;
     int rc = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = PSIM_E_HIP;

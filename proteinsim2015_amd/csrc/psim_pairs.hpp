@@ -103,7 +103,7 @@ __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
                 double r2x = pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc);
                 if (!(r2x < p.rc2)) continue;
             }
-            rinv = rsqrt(r2);
+            rinv = rsqrt_nr(r2);
             r = r2 * rinv;
         } else {
             // the reference's own arithmetic throughout (S == 3 boxes)

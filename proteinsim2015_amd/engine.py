@@ -246,8 +246,10 @@ def rsa_positions(n: int, box: float, seed: int, dmin: float = 1.0) -> np.ndarra
 class GpuSystem:
     """One system resident in HBM, advanced by the HIP kernels."""
 
+    KERNEL_PATHS = {"auto": 0, "sparse": 1, "dense": 2}
+
     def __init__(self, geometry: SystemGeometry, forces, integrator: BrownianIntegrator,
-                 noise_seed: int | None = None):
+                 noise_seed: int | None = None, kernel_path: str = "auto"):
         self.lib = _lib.load()
         if not isinstance(forces, ForceSet):
             forces = ForceSet(list(forces) if isinstance(forces, (list, tuple)) else [forces])
@@ -258,6 +260,7 @@ class GpuSystem:
         p.box = geometry.box
         p.cell_size = geometry.cell_size
         p.half_uv = 0.0
+        p.kernel_path = self.KERNEL_PATHS[kernel_path]
         p.radius = geometry.radius
         p.mass, p.kT, p.gamma, p.dt = (integrator.mass, integrator.kT, integrator.gamma,
                                        integrator.dt)

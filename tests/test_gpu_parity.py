@@ -28,13 +28,16 @@ def modules_for(plugin, cfg):
     return cls.from_config(ps.Config(cfg))
 
 
-def system_for(plugin, cfg, n, box, cell, scale, integ=None, seed=NOISE_SEED):
+PATHS = ["sparse", "dense"]
+
+
+def system_for(plugin, cfg, n, box, cell, scale, integ=None, seed=NOISE_SEED, path="auto"):
     geom = ps.SystemGeometry(n=n, box=float(box), cells_per_dim=int(scale),
                              cell_size=float(cell))
     if integ is None:
         integ = ps.BrownianIntegrator(kT=cfg.get("kT", 1.0), gamma=1.0, dt=1e-3, seed=seed)
     mods = modules_for(plugin, cfg) if isinstance(plugin, str) else plugin
-    return ps.GpuSystem(geom, mods, integ, noise_seed=seed)
+    return ps.GpuSystem(geom, mods, integ, noise_seed=seed, kernel_path=path)
 
 
 def pair_fmag(fp):
@@ -79,13 +82,14 @@ def test_cell_assignment_bit_exact(oracle, case):
         assert np.array_equal(np.sort(seg), np.sort(g["perm"][rs:re_]))
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("case", sorted(FORCE_CASES))
-def test_forces_match_reference(oracle, case):
+def test_forces_match_reference(oracle, case, path):
     plugin, mask, cfg, n, conc, scale = FORCE_CASES[case]
     g = load(f"forces_{case}.npz")
     pos, box, cell = g["pos"], int(g["box"]), int(g["cell"])
-    print(f"case={case} n={n}", flush=True)
-    with system_for(plugin, cfg, n, box, cell, scale) as sysg:
+    print(f"case={case} n={n} path={path}", flush=True)
+    with system_for(plugin, cfg, n, box, cell, scale, path=path) as sysg:
         sysg.set_state(pos)
         st, f = sysg.forces_now()
     assert st == 0
@@ -95,7 +99,8 @@ def test_forces_match_reference(oracle, case):
     assert err.max() <= FORCE_TOL, f"max rel err {err.max():.3g}"
 
 
-def test_composite_calibration_plus_lj_is_sum_of_reference_modules(oracle):
+@pytest.mark.parametrize("path", PATHS)
+def test_composite_calibration_plus_lj_is_sum_of_reference_modules(oracle, path):
     """Config 1: Calibration + LJ 12-6; the reference evaluates one plugin per
     run (defaultForceManager.cpp:56), so the oracle is the sum of two runs."""
     gc, gl = load("forces_cal.npz"), load("forces_lj6.npz")
@@ -103,7 +108,7 @@ def test_composite_calibration_plus_lj_is_sum_of_reference_modules(oracle):
     pos, box, cell, scale = gc["pos"], int(gc["box"]), int(gc["cell"]), int(gc["scale"])
     cfg = FORCE_CASES["lj6"][2]
     mods = [ps.Calibration(cutoff=2.5), ps.LennardJones.from_config(ps.Config(cfg))]
-    with system_for(mods, cfg, len(pos), box, cell, scale) as sysg:
+    with system_for(mods, cfg, len(pos), box, cell, scale, path=path) as sysg:
         sysg.set_state(pos)
         st, f = sysg.forces_now()
     assert st == 0
@@ -140,8 +145,9 @@ def traj_setup(case):
     return plugin, mask, cfg, n, scale, nsteps, g, integ
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("case", sorted(TRAJ_CASES))
-def test_single_steps_match_reference(oracle, case):
+def test_single_steps_match_reference(oracle, case, path):
     """Each reference step reproduced from the reference's state: positions
     within 1e-12 absolute (injected, replayed fastRandom noise)."""
     plugin, mask, cfg, n, scale, nsteps, g, integ = traj_setup(case)
@@ -149,7 +155,7 @@ def test_single_steps_match_reference(oracle, case):
     noise = noise_from_streams(streams, nsteps)
     box, cell = float(g["box"]), float(g["cell"])
     vf = cfg["velFreq"]
-    with system_for(plugin, cfg, n, box, cell, scale, integ=integ) as sysg:
+    with system_for(plugin, cfg, n, box, cell, scale, integ=integ, path=path) as sysg:
         for s in range(nsteps):
             print(f"case={case} step={s}", flush=True)
             memx = noise[s - 1, 3:6].T if s > 0 else None
@@ -165,13 +171,14 @@ def test_single_steps_match_reference(oracle, case):
             assert np.abs(out["frc"] - g["frc"][s + 1]).max() <= 1e-9 * fs
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("case", sorted(TRAJ_CASES))
-def test_trajectory_matches_reference(oracle, case):
+def test_trajectory_matches_reference(oracle, case, path):
     plugin, mask, cfg, n, scale, nsteps, g, integ = traj_setup(case)
     st, streams = oracle.fastrandom_streams(NOISE_SEED, n, 3 + 6 * (nsteps - 1))
     noise = noise_from_streams(streams, nsteps)
     with system_for(plugin, cfg, n, float(g["box"]), float(g["cell"]), scale,
-                    integ=integ) as sysg:
+                    integ=integ, path=path) as sysg:
         sysg.set_state(g["pos"][0], g["pos0"][0], g["vel"][0])
         assert sysg.step(nsteps, noise) == 0
         out = sysg.get_state()
@@ -179,14 +186,16 @@ def test_trajectory_matches_reference(oracle, case):
     assert np.abs(out["vel"] - g["vel"][-1]).max() <= 1e-8
 
 
-def test_in_kernel_noise_matches_oracle_philox(oracle):
+@pytest.mark.parametrize("path", PATHS)
+def test_in_kernel_noise_matches_oracle_philox(oracle, path):
     """Philox4x32-10 normals generated in the kernel = the oracle's, so a
     free-running trajectory follows the oracle's to rounding."""
     plugin, mask, cfg, n, scale, nsteps, g, integ = traj_setup("ljyuk_vel1")
     nsteps = 20
     box, cell = float(g["box"]), float(g["cell"])
     seed = 777
-    with system_for(plugin, cfg, n, box, cell, scale, integ=integ, seed=seed) as sysg:
+    with system_for(plugin, cfg, n, box, cell, scale, integ=integ, seed=seed,
+                    path=path) as sysg:
         sysg.set_state(g["pos"][0], g["pos0"][0], g["vel"][0])
         assert sysg.step(nsteps) == 0
         out = sysg.get_state()
@@ -226,7 +235,8 @@ def test_error_codes():
         assert sysg.forces_now()[0] == 7705
 
 
-def test_full_size_c3_properties(oracle):
+@pytest.mark.parametrize("path", PATHS)
+def test_full_size_c3_properties(oracle, path):
     """N = 2,097,152 Yukawa + LJ at phi 0.30: a few steps, then size-independent
     properties and a full force comparison with the oracle."""
     n = 2097152
@@ -236,7 +246,7 @@ def test_full_size_c3_properties(oracle):
     lj = ps.LennardJones.from_config(ps.Config(cfg))
     integ = ps.BrownianIntegrator(kT=1.0, gamma=1.0, dt=1e-3, seed=90210)
     print("c3: building", flush=True)
-    with ps.GpuSystem(geom, lj, integ) as sysg:
+    with ps.GpuSystem(geom, lj, integ, kernel_path=path) as sysg:
         rng = np.random.default_rng(3)
         sysg.set_state(pos, vel=rng.standard_normal((n, 3)))
         assert sysg.step(3) == 0
