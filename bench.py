@@ -34,14 +34,19 @@ CUTOFF = 3.0
 DEBYE = 0.2
 YUK_A = 20.0 / (DEBYE * math.exp(-1.0 / DEBYE))
 HBM_PEAK_GBS = 8000.0
-# algorithmic HBM bytes per particle of the fused step kernel (k_step):
-# read x, x0, F_prev, memX (4 x 24 B), write x', x0', F, memX' (4 x 24 B),
-# write cell key + atomic rank (2 x 4 B)
-STEP_KERNEL_BYTES = 200.0
-# algorithmic bytes per particle-step of the whole step (cell build and the
-# sorted position copy included), in-kernel noise:
-# 192 + 16 + 48 + 8 * ncell / N
+VALU_PEAK_LANE_OPS = 39.3e12   # gfx950: 256 CUs x 64 FP64/non-packed FP32 lanes x 2.4 GHz
+# Algorithmic HBM bytes per particle (neighbour re-reads from LDS/L2 not counted):
+#   pair kernel (k_step_dense): read x (24 B), write F (24 B)
+#   G&B kernel (k_integrate): read x, x0, F_prev, memX, F, id (124 B), write
+#     x', x0', memX', id, key, rank (84 B)
+PAIR_KERNEL_BYTES = 48.0
+INTEGRATE_KERNEL_BYTES = 208.0
+
+
 def step_bytes(ncell: int) -> float:
+    """Algorithmic bytes per particle-step of the whole step, in-kernel noise:
+    FP64 integrator state r/w 192 B, cell build 16 B, sorted position copy
+    48 B, cell-start table 8 B per cell."""
     return 192.0 + 16.0 + 48.0 + 8.0 * ncell / N
 
 
@@ -53,7 +58,30 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--path", default="auto", choices=["auto", "sparse", "dense"])
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 3 device-to-device copies of 512 MiB (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = 512 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for buf in (a, b):
+        if hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(nbytes)) != 0:
+            raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipMemset(b, 0, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:   # device to device
+            raise SystemExit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": 3}))
 
 
 def cpu_baseline(pos_exact, box_exact, steps):
@@ -109,8 +137,27 @@ def traffic_from_profiles():
         return None
 
 
+def valu_from_profiles(pair_ms):
+    """VALU lane-op rate of the pair kernel from a committed rocprofv3 --pmc
+    run (SQ_INSTS_VALU x 64 lanes per launch / measured launch time)."""
+    path = os.path.join(ROOT, "profiles", "k_step_c3_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        insts = d["valu_insts_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+    rate = insts * 64 / (pair_ms * 1e-3)
+    return {"achieved_lane_ops_per_s": rate, "peak": VALU_PEAK_LANE_OPS,
+            "frac": rate / VALU_PEAK_LANE_OPS,
+            "note": "SQ_INSTS_VALU x 64 counts issued lanes incl. masked ones"}
+
+
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not a measured path)", file=sys.stderr)
@@ -149,11 +196,16 @@ def main():
         raise SystemExit(f"bench: physics error {code} in the timed region")
     elapsed = t1 - t0
     value = N * args.steps / elapsed
-    step_ms = kt["force_ms"] / max(1, kt["force_launches"])
-    achieved = STEP_KERNEL_BYTES * N / (step_ms * 1e-3) / 1e9
+    pair_ms = kt["force_ms"] / max(1, kt["force_launches"])
+    integ_ms = kt["integrate_ms"] / max(1, kt["integrate_launches"]) \
+        if kt["integrate_launches"] else 0.0
+    build_ms = kt["build_ms"] / max(1, kt["build_launches"])
+    pair_bytes = PAIR_KERNEL_BYTES if kt["integrate_launches"] else \
+        PAIR_KERNEL_BYTES + INTEGRATE_KERNEL_BYTES - 48.0
+    achieved = pair_bytes * N / (pair_ms * 1e-3) / 1e9
     wall_step_bytes = step_bytes(geom.ncell)
-    print(f"bench: {args.steps} steps in {elapsed:.3f}s; k_step {step_ms:.3f} ms, "
-          f"build {kt['build_ms'] / max(1, kt['build_launches']):.3f} ms", file=sys.stderr,
+    print(f"bench: {args.steps} steps in {elapsed:.3f}s; pair kernel {pair_ms:.3f} ms, "
+          f"G&B kernel {integ_ms:.3f} ms, cell rebuild {build_ms:.3f} ms", file=sys.stderr,
           flush=True)
     sysg.close()
     cpu = None if args.no_cpu_baseline else cpu_baseline(pos, geom.box, args.cpu_steps)
@@ -181,15 +233,21 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_step (fused 27-cell forces + G&B + cell key)",
+            "kernel": "k_step_dense (27-cell LDS-staged pair forces)" if kt["integrate_launches"]
+                      else "k_step (27-cell pair forces + fused G&B)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(),
-            "bytes_per_particle": STEP_KERNEL_BYTES,
-            "avg_launch_ms": step_ms,
+            "bytes_per_particle": pair_bytes,
+            "avg_launch_ms": pair_ms,
+            "g_and_b_kernel_ms": integ_ms,
+            "cell_rebuild_ms": build_ms,
+            "whole_step_bytes_per_particle": wall_step_bytes,
+            "whole_step_achieved_gbs": wall_step_bytes * value / 1e9,
             "whole_step_frac": wall_step_bytes * value / (HBM_PEAK_GBS * 1e9),
+            "valu": valu_from_profiles(pair_ms),
         },
         "cpu_baseline": cpu,
     }

@@ -55,7 +55,10 @@ class Reference:
 
     @staticmethod
     def plugin(name):
-        return os.path.join(REF_DIR, PLUGINS[name]).encode()
+        """A reference plugin by name, or any plugin .so by path."""
+        if name in PLUGINS:
+            return os.path.join(REF_DIR, PLUGINS[name]).encode()
+        return os.path.abspath(name).encode()
 
     def fastrandom(self, seed, n, ndraws):
         out = np.empty(n * ndraws)

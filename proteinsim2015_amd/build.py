@@ -69,7 +69,26 @@ def build_product(verbose: bool = False) -> str:
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs, "-lgomp"],
          verbose)
     os.replace(out, LIB)
+    build_plugins(verbose)
     return LIB
+
+
+PLUGINS = {"LJ": 2, "AO": 4, "Cal": 1}   # file name (as the reference's) -> module bit
+
+
+def build_plugins(verbose: bool = False) -> None:
+    """PSim::IForce plugins (host C++, linked to libpsim_gpu.so)."""
+    src = os.path.join(PLUGIN_DIR, "psim_force_plugin.cpp")
+    for name, mask in PLUGINS.items():
+        out = os.path.join(PLUGIN_DIR, f"{name}.so")
+        _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I", INCLUDE,
+              f"-DPSIM_PLUGIN_MASK={mask}", src, "-o", out + ".tmp", "-L", PKG, "-lpsim_gpu",
+              "-Wl,-rpath,$ORIGIN/.."], verbose)
+        os.replace(out + ".tmp", out)
+    demo = os.path.join(PLUGIN_DIR, "psim_host_demo")
+    _run(["g++", "-std=c++17", "-O2", "-fopenmp", "-Wall", "-I", INCLUDE,
+          os.path.join(PLUGIN_DIR, "psim_host_demo.cpp"), "-o", demo + ".tmp", "-ldl"], verbose)
+    os.replace(demo + ".tmp", demo)
 
 
 def build_oracle(verbose: bool = False) -> None:

@@ -62,6 +62,7 @@ struct StepArgs {
 // G&B epilogue for the particle in slot s with total force F
 // (brownianIntegrator.cpp:191-356, particle.cpp:61-99); writes the next state
 // at slot s, the new cell key and an atomic per-cell rank.
+template <bool WRITE_F>
 __device__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int s,
                               double fx, double fy, double fz)
 {
@@ -133,7 +134,7 @@ __device__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int
         bad |= (xw[d] < 0.0) || (xw[d] >= g.L);
         a.npos[d * N + s] = xw[d];
         a.npos0[d * N + s] = x0n;
-        a.nfrc[d * N + s] = F[d];
+        if (WRITE_F) a.nfrc[d * N + s] = F[d];
         a.nmem[d * N + s] = M[d];
     }
     a.nid[s] = o;
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_step(Geo g, Pot p, GB gb, double rad
     else
         gather_force<MASK, false>(g, p, src, tab, s, xi, yi, zi, radius,
                                   hits + threadIdx.x, fx, fy, fz, a.err, orig);
-    integrate_one(g, gb, a, s, fx, fy, fz);
+    integrate_one<true>(g, gb, a, s, fx, fy, fz);
 }
 
 // Cell-per-workgroup forces for the step (dense systems, S >= 4): the total
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(256) void k_integrate(Geo g, GB gb, StepArgs a)
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= g.n) return;
     const int N = g.n;
-    integrate_one(g, gb, a, s, a.nfrc[s], a.nfrc[N + s], a.nfrc[2 * N + s]);
+    integrate_one<false>(g, gb, a, s, a.nfrc[s], a.nfrc[N + s], a.nfrc[2 * N + s]);
 }
 
 // Forces only at the current state, written by original index ([3][N]).
