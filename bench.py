@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--path", default="auto", choices=["auto", "sparse", "dense"])
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 512-replica batched-ensemble measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 device-to-device copies of 512 MiB (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -208,6 +210,11 @@ def main():
           f"G&B kernel {integ_ms:.3f} ms, cell rebuild {build_ms:.3f} ms", file=sys.stderr,
           flush=True)
     sysg.close()
+    ensemble = None
+    if not args.no_ensemble:
+        from proteinsim2015_amd import bench_ensemble
+        ensemble = bench_ensemble.run(512, steps=500, baseline_steps=20)
+        print(f"bench: ensemble speedup {ensemble['speedup']:.2f}x", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline(pos, geom.box, args.cpu_steps)
     line = {
         "metric": "particle-steps/s at N=2M",
@@ -250,6 +257,7 @@ def main():
             "valu": valu_from_profiles(pair_ms),
         },
         "cpu_baseline": cpu,
+        "batched_ensemble": ensemble,
     }
     print(json.dumps(line))
 

@@ -158,6 +158,50 @@ int psim_gpu_kernel_times(psim_ctx *ctx, psim_kernel_times *out, int reset);
  * against it. */
 void *psim_gpu_stream(psim_ctx *ctx);
 
+/* ---- replica ensemble ---------------------------------------------------
+ * Many independent systems of uneven N, box and force strength advanced
+ * together: one launch per kernel stage per step for all replicas, and the
+ * step sequence can be captured once in a hipGraph and replayed.  Replicas
+ * share the integrator parameters and force modules of `common` (kT, gamma,
+ * mass, dt, velFreq, seed, radius, force_mask); geometry and force strengths
+ * are per replica.  Arrays are AoS3 concatenated in replica order (replica r
+ * owns original ids [sum_{q<r} n_q, sum_{q<=r} n_q)).  No reference
+ * counterpart: the reference runs one system per process. */
+typedef struct psim_replica {
+    int n_particles;
+    int cells_per_dim;
+    double box;
+    double cell_size;
+    double cutoff;
+    double lj_well_depth;
+    double yukawa_strength;
+    double debye_length;
+    double ao_well_depth;
+    int lj_num;
+    int reserved;
+} psim_replica;
+
+typedef struct psim_ens psim_ens;
+
+int psim_ens_create(const psim_params *common, const psim_replica *reps, int nrep,
+                    psim_ens **out);
+void psim_ens_destroy(psim_ens *ens);
+int psim_ens_total_particles(psim_ens *ens);
+/* vel may be NULL; current_time == 0 makes the next step a first step */
+int psim_ens_set_state(psim_ens *ens, const double *pos, const double *vel,
+                       double current_time);
+int psim_ens_get_state(psim_ens *ens, double *pos, double *pos0, double *vel, double *frc);
+/* steps_per_graph > 0: psim_ens_step replays a captured graph of that many
+ * steps (captured on first use); 0: eager launches */
+int psim_ens_set_graph(psim_ens *ens, int steps_per_graph);
+/* nsteps with in-kernel Philox noise; asynchronous (see psim_ens_sync) */
+int psim_ens_step(psim_ens *ens, int nsteps);
+int psim_ens_sync(psim_ens *ens);
+int psim_ens_last_error(psim_ens *ens, int *code, int *i, int *j);
+/* measured choice of the pair kernel's workgroup size (64/128/256); the
+ * trial steps advance the ensemble */
+int psim_ens_autotune(psim_ens *ens, int trial_steps, int *chosen_block, float *ms_per_step);
+
 /* Seeded random sequential addition of n spheres with pair distance >= dmin
  * in a periodic cube [0, box)^3 (host, O(n)).  Returns 0 or PSIM_ERR_INIT. */
 int psim_init_rsa(int n, double box, double dmin, unsigned long long seed,

@@ -97,6 +97,8 @@ class Oracle:
                              C.c_int, C.c_void_p, C.c_ulonglong,
                              C.POINTER(C.c_ulonglong), _ip, C.c_int]
         L.po_run.restype = C.c_int
+        L.po_run_ids.argtypes = L.po_run.argtypes + [C.c_int]
+        L.po_run_ids.restype = C.c_int
         L.po_temperature.argtypes = [C.c_int, _dp]
         L.po_temperature.restype = C.c_double
         L.po_pair_histogram.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
@@ -203,7 +205,7 @@ class Oracle:
 
     # -- full run -------------------------------------------------------------
     def run(self, fp, c, state, nsteps, box, S, cell_size, radius=0.5,
-            vel_freq=1000, noise=None, seed=0, half_uv=None, nthreads=0):
+            vel_freq=1000, noise=None, seed=0, half_uv=None, nthreads=0, id0=0):
         """Advance ``state`` (dict of AoS3 arrays + counters) in place."""
         n = state["pos"].shape[0]
         if half_uv is None:
@@ -218,13 +220,13 @@ class Oracle:
         if noise is not None:
             nz = _f64(noise).reshape(-1)
             assert nz.size == nsteps * 6 * n
-        st = self.lib.po_run(C.byref(fp), C.byref(c), n, float(box), float(half_uv),
-                             int(S), float(cell_size), float(radius), int(vel_freq),
-                             C.byref(vc), C.byref(t), arrs["pos"], arrs["pos0"],
-                             arrs["vel"], arrs["frc"], arrs["frc0"], arrs["memx"],
-                             int(nsteps),
-                             None if nz is None else nz.ctypes.data_as(C.c_void_p),
-                             int(seed), C.byref(si), err, int(nthreads))
+        st = self.lib.po_run_ids(C.byref(fp), C.byref(c), n, float(box), float(half_uv),
+                                 int(S), float(cell_size), float(radius), int(vel_freq),
+                                 C.byref(vc), C.byref(t), arrs["pos"], arrs["pos0"],
+                                 arrs["vel"], arrs["frc"], arrs["frc0"], arrs["memx"],
+                                 int(nsteps),
+                                 None if nz is None else nz.ctypes.data_as(C.c_void_p),
+                                 int(seed), C.byref(si), err, int(nthreads), int(id0))
         for k, v in arrs.items():
             state[k] = v.reshape(-1, 3)
         state["vel_counter"], state["time"], state["step_index"] = vc.value, t.value, si.value

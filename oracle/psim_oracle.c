@@ -632,6 +632,20 @@ int po_run(const po_force_params *fp, const po_gb_coeffs *c,
            unsigned long long seed, unsigned long long *step_index,
            int *err_ij, int nthreads)
 {
+    return po_run_ids(fp, c, n, box, half_uv, S, cell_size, radius, vel_freq, vel_counter,
+                      current_time, pos, pos0, vel, frc, frc0, memx, nsteps, noise, seed,
+                      step_index, err_ij, nthreads, 0);
+}
+
+int po_run_ids(const po_force_params *fp, const po_gb_coeffs *c,
+               int n, double box, double half_uv, int S, double cell_size,
+               double radius, int vel_freq, int *vel_counter,
+               double *current_time,
+               double *pos, double *pos0, double *vel, double *frc, double *frc0,
+               double *memx, int nsteps, const double *noise,
+               unsigned long long seed, unsigned long long *step_index,
+               int *err_ij, int nthreads, int id0)
+{
     int ncell = S * S * S;
     int *key = (int *)malloc(sizeof(int) * (size_t)n);
     int *perm = (int *)malloc(sizeof(int) * (size_t)n);
@@ -660,7 +674,7 @@ int po_run(const po_force_params *fp, const po_gb_coeffs *c,
         if (noise) {
             src = noise + (size_t)s * 6 * n;
         } else {
-            po_philox_normals(seed, *step_index, 0, n, nz);
+            po_philox_normals(seed, *step_index, id0, n, nz);
             src = nz;
         }
         for (int i = 0; i < n; i++) {

@@ -135,6 +135,15 @@ int po_run(const po_force_params *fp, const po_gb_coeffs *c,
            double *memx, int nsteps, const double *noise,
            unsigned long long seed, unsigned long long *step_index,
            int *err_ij, int nthreads);
+/* po_run with the in-kernel noise keyed by original ids id0 .. id0+n-1 */
+int po_run_ids(const po_force_params *fp, const po_gb_coeffs *c,
+               int n, double box, double half_uv, int S, double cell_size,
+               double radius, int vel_freq, int *vel_counter,
+               double *current_time,
+               double *pos, double *pos0, double *vel, double *frc, double *frc0,
+               double *memx, int nsteps, const double *noise,
+               unsigned long long seed, unsigned long long *step_index,
+               int *err_ij, int nthreads, int id0);
 
 /* analysisGeneral.cpp:26-44: sum v^2 / 3 / N. */
 double po_temperature(int n, const double *vel);

@@ -31,6 +31,14 @@ class PsimParams(C.Structure):
     ]
 
 
+class PsimReplica(C.Structure):
+    _fields_ = [("n_particles", C.c_int), ("cells_per_dim", C.c_int),
+                ("box", C.c_double), ("cell_size", C.c_double), ("cutoff", C.c_double),
+                ("lj_well_depth", C.c_double), ("yukawa_strength", C.c_double),
+                ("debye_length", C.c_double), ("ao_well_depth", C.c_double),
+                ("lj_num", C.c_int), ("reserved", C.c_int)]
+
+
 class KernelTimes(C.Structure):
     _fields_ = [("force_ms", C.c_double), ("force_launches", C.c_int),
                 ("integrate_ms", C.c_double), ("integrate_launches", C.c_int),
@@ -46,6 +54,9 @@ EXPORTS = (
     "psim_gpu_get_cells", "psim_gpu_last_error", "psim_gpu_clear_error",
     "psim_gpu_sync", "psim_gpu_set_profiling", "psim_gpu_kernel_times",
     "psim_gpu_stream", "psim_init_rsa",
+    "psim_ens_create", "psim_ens_destroy", "psim_ens_total_particles", "psim_ens_set_state",
+    "psim_ens_get_state", "psim_ens_set_graph", "psim_ens_step", "psim_ens_sync",
+    "psim_ens_last_error", "psim_ens_autotune",
 )
 
 _vp = C.c_void_p
@@ -73,6 +84,17 @@ _SIGS = {
     "psim_gpu_kernel_times": ([_vp, C.POINTER(KernelTimes), C.c_int], C.c_int),
     "psim_gpu_stream": ([_vp], _vp),
     "psim_init_rsa": ([C.c_int, C.c_double, C.c_double, C.c_ulonglong, _dp], C.c_int),
+    "psim_ens_create": ([C.POINTER(PsimParams), C.POINTER(PsimReplica), C.c_int,
+                         C.POINTER(_vp)], C.c_int),
+    "psim_ens_destroy": ([_vp], None),
+    "psim_ens_total_particles": ([_vp], C.c_int),
+    "psim_ens_set_state": ([_vp, _dp, _dp, C.c_double], C.c_int),
+    "psim_ens_get_state": ([_vp, _dp, _dp, _dp, _dp], C.c_int),
+    "psim_ens_set_graph": ([_vp, C.c_int], C.c_int),
+    "psim_ens_step": ([_vp, C.c_int], C.c_int),
+    "psim_ens_sync": ([_vp], C.c_int),
+    "psim_ens_last_error": ([_vp, _ip, _ip, _ip], C.c_int),
+    "psim_ens_autotune": ([_vp, C.c_int, _ip, C.POINTER(C.c_float)], C.c_int),
 }
 
 _LIB = None

@@ -33,7 +33,7 @@ ARCH = os.environ.get("PSIM_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-result",
+    "-Wall", "-Wno-unused-result", "-Wno-unused-function",
 ]
 
 
@@ -55,7 +55,7 @@ def build_product(verbose: bool = False) -> str:
     objs = []
     tmp = os.path.join(PKG, "_obj")
     os.makedirs(tmp, exist_ok=True)
-    for src in ("psim_engine.hip",):
+    for src in ("psim_engine.hip", "psim_ensemble.hip"):
         obj = os.path.join(tmp, src + ".o")
         _run([hipcc, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c",
               os.path.join(CSRC, src), "-o", obj], verbose)

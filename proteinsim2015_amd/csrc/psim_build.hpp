@@ -1,0 +1,150 @@
+// psim_build.hpp -- the per-step cell rebuild (hash, counting sort,
+// cell-start table, canonical reorder of the state), kernels shared by the
+// single-system engine and the replica ensemble.
+//
+// Replaces system::hashParticles / sortParticles / clearCells /
+// reorderParticles (systemHandling.cpp:33-85): instead of an O(N log N)
+// comparison sort of (hash, index) tuples every step, each particle's new
+// cell is counted with a wave-aggregated atomic during the step, the counts
+// are scanned into a start table, and the state is gathered into cell order.
+// Within a cell the order is the previous slot order (a stable counting
+// sort), which makes every run reproducible bit for bit.
+#pragma once
+
+#include "psim_device.hpp"
+
+namespace psim {
+namespace {
+
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Cell hash + atomic count for an arbitrary order (initial build).
+__global__ void k_hash_count(Geo g, const double *pos, int *key, int *arank, int *count)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.n) return;
+    const int N = g.n;
+    int k = g.c0 + cell_1d(pos[s], g.cell, g.S) + g.S * cell_1d(pos[N + s], g.cell, g.S)
+          + g.S2 * cell_1d(pos[2 * N + s], g.cell, g.S);
+    key[s] = k;
+    arank[s] = wave_aggregated_rank(count, k);
+}
+
+// Exclusive scan over a 1024-thread block of one value per thread.
+__device__ int block_exclusive_scan(int v, int *total)
+{
+    __shared__ int wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        int t = lane < kScanThreads / 64 ? wsum[lane] : 0;
+#pragma unroll
+        for (int d = 1; d < kScanThreads / 64; d <<= 1) {
+            int y = __shfl_up(t, d, 64);
+            if (lane >= d) t += y;
+        }
+        if (lane < kScanThreads / 64) wsum[lane] = t;
+    }
+    __syncthreads();
+    int pre = (w > 0 ? wsum[w - 1] : 0) + x - v;
+    *total = wsum[kScanThreads / 64 - 1];
+    __syncthreads();
+    return pre;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int *count, int ncell,
+                                                              int *bsum)
+{
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < ncell) v += count[base + k];
+    int total;
+    block_exclusive_scan(v, &total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_bsums(int *bsum, int nb)
+{
+    int carry = 0;
+    for (int base = 0; base < nb; base += kScanThreads) {
+        int i = base + threadIdx.x;
+        int v = i < nb ? bsum[i] : 0;
+        int total;
+        int pre = block_exclusive_scan(v, &total);
+        if (i < nb) bsum[i] = carry + pre;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(int *count, int ncell,
+                                                             const int *bsum,
+                                                             int *cstart, int n)
+{
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int c[kScanItems];
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        c[k] = (base + k < ncell) ? count[base + k] : 0;
+        v += c[k];
+    }
+    int total;
+    int pre = block_exclusive_scan(v, &total) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < ncell) {
+            cstart[base + k] = pre;
+            count[base + k] = 0;
+        }
+        pre += c[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
+}
+
+__global__ void k_scatter(int n, const int *key, const int *arank, const int *cstart,
+                          int *perm)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    perm[cstart[key[s]] + arank[s]] = s;
+}
+
+// Canonical order within a cell = ascending source slot (stable counting
+// sort); gathers the state into the sorted arrays.
+__global__ void k_move(int n, const int *perm, const int *key, const int *cstart,
+                       const double *npos, const double *npos0, const double *nfrc,
+                       const double *nmem, const int *nid, double *pos, double *pos0,
+                       double *frc, double *mem, int *id)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int s = perm[t];
+    const int c = key[s];
+    const int b = cstart[c], e = cstart[c + 1];
+    int rank = 0;
+    for (int u = b; u < e; ++u) rank += (perm[u] < s);
+    const int dst = b + rank;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        pos[d * n + dst] = npos[d * n + s];
+        pos0[d * n + dst] = npos0[d * n + s];
+        frc[d * n + dst] = nfrc[d * n + s];
+        mem[d * n + dst] = nmem[d * n + s];
+    }
+    id[dst] = nid[s];
+}
+
+
+}  // namespace
+}  // namespace psim

@@ -87,7 +87,8 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
 {
     const int N = g.n, S = g.S;
     const int lane = threadIdx.x;
-    const int cx = c % S, cy = (c / S) % S, cz = c / g.S2;
+    const int cl = c - g.c0;   // cell index within this system
+    const int cx = cl % S, cy = (cl / S) % S, cz = cl / g.S2;
     const int home0 = cstart[c];
     const int ni = cstart[c + 1] - home0;
     if (ni == 0) return;   // uniform over the workgroup
@@ -100,7 +101,7 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
         if (rx < 0) { rx += S; sxv = -g.L; } else if (rx >= S) { rx -= S; sxv = g.L; }
         if (ry < 0) { ry += S; syv = -g.L; } else if (ry >= S) { ry -= S; syv = g.L; }
         if (rz < 0) { rz += S; szv = -g.L; } else if (rz >= S) { rz -= S; szv = g.L; }
-        const int h = rx + S * ry + g.S2 * rz;
+        const int h = g.c0 + rx + S * ry + g.S2 * rz;
         const int b = cstart[h];
         cnt = cstart[h + 1] - b;
         sh.cbase[lane] = b;

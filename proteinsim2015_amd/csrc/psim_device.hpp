@@ -30,7 +30,9 @@ struct Geo {
     double half_uv;   // L / 2 as used by unitVectorAdv (utilities.cpp:124)
     double cell;      // cell edge used by the hash
     int exact_mi;     // 1: per-pair reference minimum image (S < 4)
-    int pad;
+    int c0;           // offset of this system's cells in the cell-index space
+                      // (0 for one system; replicas of an ensemble are
+                      // stacked, each owning S^3 consecutive cells)
 };
 
 // Pair-force constants (LJPotential.cpp:34-55, AOPotential.cpp:40-55).

@@ -30,384 +30,12 @@
 #include <vector>
 
 #include "psim_gpu.h"
-#include "psim_dense.hpp"
-#include "psim_pairs.hpp"
+#include "psim_build.hpp"
+#include "psim_params.hpp"
+#include "psim_step.hpp"
 
 using namespace psim;
 
-namespace {
-
-constexpr int kScanThreads = 1024;
-// mean particles per cell above which the cell-per-workgroup kernel is used
-constexpr double kDenseMinOccupancy = 2.0;
-constexpr int kScanItems = 4;
-constexpr int kScanTile = kScanThreads * kScanItems;
-
-// ---------------------------------------------------------------------------
-// Kernels
-// ---------------------------------------------------------------------------
-struct StepArgs {
-    const double *pos, *pos0, *frc, *mem;
-    const int *id, *cstart;
-    double *npos, *npos0, *nfrc, *nmem;
-    int *nid;
-    double *vel;         // [3][N] by original index
-    int *key, *arank, *count;
-    DevErr *err;
-    const double *noise; // [6][N] by original index, or null -> Philox
-    unsigned long long seed, step;
-    int first, do_vel;
-};
-
-// G&B epilogue for the particle in slot s with total force F
-// (brownianIntegrator.cpp:191-356, particle.cpp:61-99); writes the next state
-// at slot s, the new cell key and an atomic per-cell rank.
-template <bool WRITE_F>
-__device__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int s,
-                              double fx, double fy, double fz)
-{
-    const int N = g.n;
-    const int o = a.id[s];
-    double z[6];
-    if (a.noise) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) z[k] = a.noise[(size_t)k * N + o];
-    } else {
-        philox_normals6(a.seed, a.step, (uint32_t)o, z);
-    }
-    const double xc[3] = {a.pos[s], a.pos[N + s], a.pos[2 * N + s]};
-    const double F[3] = {fx, fy, fz};
-    double x0c[3], xn[3], M[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) x0c[d] = a.pos0[d * N + s];
-    if (a.first) {
-        // firstStep :200-218
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            M[d] = z[3 + d];
-            double v = a.vel[(size_t)d * N + o];
-            xn[d] = xc[d] + (v * gb.coeff1 * gb.dt)
-                  + (F[d] * gb.coeff3 * gb.dt * gb.dt * gb.m) + (gb.sig1 * M[d]);
-        }
-    } else {
-        // normalStep :238-274
-        double Mo[3], Fo[3], Cr[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            Mo[d] = a.mem[d * N + s];
-            Fo[d] = a.frc[d * N + s];
-            Cr[d] = gb.sig2 * ((gb.corr * Mo[d]) + (gb.dev * z[d]));
-            M[d] = z[3 + d];
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double q = (gb.c0 * xc[d]);
-            q -= (gb.coeff0 * x0c[d]);
-            q += (gb.c1 * F[d]);
-            q += (gb.c2 * (F[d] - Fo[d]));
-            q += (gb.sig1 * M[d]) + (gb.coeff0 * Cr[d]);
-            xn[d] = q;
-        }
-        if (a.do_vel) {
-            // velocityStep :305-354
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                double dx0 = xc[d] - x0c[d];
-                double xN = safe_mod(xn[d], g.L);
-                double x0 = safe_mod0(xc[d], xN, g.L);
-                double dx = xN - x0;
-                double vv = dx + dx0;
-                vv += (gb.g2 * F[d]);
-                vv -= (gb.g3 * (F[d] - Fo[d]));
-                vv += (Cr[d] - gb.sig1 * M[d]);
-                vv *= gb.g4;
-                a.vel[(size_t)d * N + o] = vv;
-            }
-        }
-    }
-    bool bad = false;
-    double xw[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        xw[d] = safe_mod(xn[d], g.L);
-        double x0n = safe_mod0(xc[d], xw[d], g.L);
-        bad |= (xw[d] < 0.0) || (xw[d] >= g.L);
-        a.npos[d * N + s] = xw[d];
-        a.npos0[d * N + s] = x0n;
-        if (WRITE_F) a.nfrc[d * N + s] = F[d];
-        a.nmem[d * N + s] = M[d];
-    }
-    a.nid[s] = o;
-    if (bad) report_error(a.err, 7703, o, -1);
-    int k = cell_1d(xw[0], g.cell, g.S) + g.S * cell_1d(xw[1], g.cell, g.S)
-          + g.S2 * cell_1d(xw[2], g.cell, g.S);
-    a.key[s] = k;
-    a.arank[s] = wave_aggregated_rank(a.count, k);
-}
-
-// Thread-per-particle step (sparse systems and S < 4).
-template <int MASK>
-__global__ __launch_bounds__(kBlock) void k_step(Geo g, Pot p, GB gb, double radius,
-                                                 StepArgs a)
-{
-    __shared__ unsigned hits[kHitCap * kBlock];
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= g.n) return;
-    const int N = g.n;
-    const double xi = a.pos[s], yi = a.pos[N + s], zi = a.pos[2 * N + s];
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    SoAPos src{a.pos, N, radius};
-    StartTab tab{a.cstart};
-    Orig orig{a.id, 1, 0};
-    if (g.exact_mi)
-        gather_force<MASK, true>(g, p, src, tab, s, xi, yi, zi, radius,
-                                 hits + threadIdx.x, fx, fy, fz, a.err, orig);
-    else
-        gather_force<MASK, false>(g, p, src, tab, s, xi, yi, zi, radius,
-                                  hits + threadIdx.x, fx, fy, fz, a.err, orig);
-    integrate_one<true>(g, gb, a, s, fx, fy, fz);
-}
-
-// Cell-per-workgroup forces for the step (dense systems, S >= 4): the total
-// force on each particle goes to nfrc[slot], where k_integrate picks it up.
-// (Fusing the G&B epilogue here would run it on n_i of the workgroup's lanes,
-// latency-bound; the separate kernel runs it at full occupancy.)
-template <int MASK>
-__global__ __launch_bounds__(kDenseThreads) void k_step_dense(Geo g, Pot p, double radius,
-                                                              const double *pos,
-                                                              const int *id,
-                                                              const int *cstart,
-                                                              double *nfrc, DevErr *err,
-                                                              int per_xcd)
-{
-    __shared__ DenseShared sh;
-    const int c = xcd_cell(blockIdx.x, per_xcd);
-    if (c >= g.ncell) return;
-    const int home0 = cstart[c];
-    const int N = g.n;
-    dense_cell_forces<MASK>(g, p, radius, pos, cstart, id, err, c, sh,
-                            [=](int i, double Fx, double Fy, double Fz) {
-                                const int s = home0 + i;
-                                nfrc[s] = Fx;
-                                nfrc[N + s] = Fy;
-                                nfrc[2 * N + s] = Fz;
-                            });
-}
-
-// G&B step for every slot with the force already in nfrc (dense path).
-__global__ __launch_bounds__(256) void k_integrate(Geo g, GB gb, StepArgs a)
-{
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= g.n) return;
-    const int N = g.n;
-    integrate_one<false>(g, gb, a, s, a.nfrc[s], a.nfrc[N + s], a.nfrc[2 * N + s]);
-}
-
-// Forces only at the current state, written by original index ([3][N]).
-template <int MASK>
-__global__ __launch_bounds__(kBlock) void k_forces(Geo g, Pot p, double radius,
-                                                   const double *pos, const int *id,
-                                                   const int *cstart, double *fout,
-                                                   DevErr *err)
-{
-    __shared__ unsigned hits[kHitCap * kBlock];
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= g.n) return;
-    const int N = g.n;
-    const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    SoAPos src{pos, N, radius};
-    StartTab tab{cstart};
-    Orig orig{id, 1, 0};
-    if (g.exact_mi)
-        gather_force<MASK, true>(g, p, src, tab, s, xi, yi, zi, radius,
-                                 hits + threadIdx.x, fx, fy, fz, err, orig);
-    else
-        gather_force<MASK, false>(g, p, src, tab, s, xi, yi, zi, radius,
-                                  hits + threadIdx.x, fx, fy, fz, err, orig);
-    const int o = id[s];
-    fout[o] = fx;
-    fout[N + o] = fy;
-    fout[2 * N + o] = fz;
-}
-
-template <int MASK>
-__global__ __launch_bounds__(kDenseThreads) void k_forces_dense(Geo g, Pot p, double radius,
-                                                                const double *pos,
-                                                                const int *id,
-                                                                const int *cstart,
-                                                                double *fout, DevErr *err,
-                                                                int per_xcd)
-{
-    __shared__ DenseShared sh;
-    const int c = xcd_cell(blockIdx.x, per_xcd);
-    if (c >= g.ncell) return;
-    const int home0 = cstart[c];
-    const int N = g.n;
-    dense_cell_forces<MASK>(g, p, radius, pos, cstart, id, err, c, sh,
-                            [=](int i, double Fx, double Fy, double Fz) {
-                                const int o = id[home0 + i];
-                                fout[o] = Fx;
-                                fout[N + o] = Fy;
-                                fout[2 * N + o] = Fz;
-                            });
-}
-
-// Forces from the reference's host arrays (IForce::getAcceleration over the
-// whole system): sorted4, particleHashIndex (hash, orig), cellStartEnd.
-template <int MASK>
-__global__ __launch_bounds__(kBlock) void k_forces_table(Geo g, Pot p,
-                                                         const double *sorted4,
-                                                         const int *hash_index,
-                                                         const int *cse, double *fout,
-                                                         DevErr *err)
-{
-    __shared__ unsigned hits[kHitCap * kBlock];
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= g.n) return;
-    AoS4Pos src{sorted4};
-    PairTab tab{cse};
-    Orig orig{hash_index, 2, 1};
-    const double xi = src.x(s), yi = src.y(s), zi = src.z(s), ri = src.rad(s);
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    if (g.exact_mi)
-        gather_force<MASK, true>(g, p, src, tab, s, xi, yi, zi, ri,
-                                 hits + threadIdx.x, fx, fy, fz, err, orig);
-    else
-        gather_force<MASK, false>(g, p, src, tab, s, xi, yi, zi, ri,
-                                  hits + threadIdx.x, fx, fy, fz, err, orig);
-    const int o = orig(s);
-    fout[3 * (size_t)o] = fx;   // particleForce[3*orig + k] (LJPotential.cpp:179-181)
-    fout[3 * (size_t)o + 1] = fy;
-    fout[3 * (size_t)o + 2] = fz;
-}
-
-// Cell hash + atomic count for an arbitrary order (initial build).
-__global__ void k_hash_count(Geo g, const double *pos, int *key, int *arank, int *count)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= g.n) return;
-    const int N = g.n;
-    int k = cell_1d(pos[s], g.cell, g.S) + g.S * cell_1d(pos[N + s], g.cell, g.S)
-          + g.S2 * cell_1d(pos[2 * N + s], g.cell, g.S);
-    key[s] = k;
-    arank[s] = wave_aggregated_rank(count, k);
-}
-
-// Exclusive scan over a 1024-thread block of one value per thread.
-__device__ int block_exclusive_scan(int v, int *total)
-{
-    __shared__ int wsum[kScanThreads / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    if (w == 0) {
-        int t = lane < kScanThreads / 64 ? wsum[lane] : 0;
-#pragma unroll
-        for (int d = 1; d < kScanThreads / 64; d <<= 1) {
-            int y = __shfl_up(t, d, 64);
-            if (lane >= d) t += y;
-        }
-        if (lane < kScanThreads / 64) wsum[lane] = t;
-    }
-    __syncthreads();
-    int pre = (w > 0 ? wsum[w - 1] : 0) + x - v;
-    *total = wsum[kScanThreads / 64 - 1];
-    __syncthreads();
-    return pre;
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int *count, int ncell,
-                                                              int *bsum)
-{
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    int v = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < ncell) v += count[base + k];
-    int total;
-    block_exclusive_scan(v, &total);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_scan_bsums(int *bsum, int nb)
-{
-    int carry = 0;
-    for (int base = 0; base < nb; base += kScanThreads) {
-        int i = base + threadIdx.x;
-        int v = i < nb ? bsum[i] : 0;
-        int total;
-        int pre = block_exclusive_scan(v, &total);
-        if (i < nb) bsum[i] = carry + pre;
-        carry += total;
-    }
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply(int *count, int ncell,
-                                                             const int *bsum,
-                                                             int *cstart, int n)
-{
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    int c[kScanItems];
-    int v = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        c[k] = (base + k < ncell) ? count[base + k] : 0;
-        v += c[k];
-    }
-    int total;
-    int pre = block_exclusive_scan(v, &total) + bsum[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < ncell) {
-            cstart[base + k] = pre;
-            count[base + k] = 0;
-        }
-        pre += c[k];
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
-}
-
-__global__ void k_scatter(int n, const int *key, const int *arank, const int *cstart,
-                          int *perm)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    perm[cstart[key[s]] + arank[s]] = s;
-}
-
-// Canonical order within a cell = ascending source slot (stable counting
-// sort); gathers the state into the sorted arrays.
-__global__ void k_move(int n, const int *perm, const int *key, const int *cstart,
-                       const double *npos, const double *npos0, const double *nfrc,
-                       const double *nmem, const int *nid, double *pos, double *pos0,
-                       double *frc, double *mem, int *id)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int s = perm[t];
-    const int c = key[s];
-    const int b = cstart[c], e = cstart[c + 1];
-    int rank = 0;
-    for (int u = b; u < e; ++u) rank += (perm[u] < s);
-    const int dst = b + rank;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        pos[d * n + dst] = npos[d * n + s];
-        pos0[d * n + dst] = npos0[d * n + s];
-        frc[d * n + dst] = nfrc[d * n + s];
-        mem[d * n + dst] = nmem[d * n + s];
-    }
-    id[dst] = nid[s];
-}
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // Context
@@ -455,117 +83,7 @@ struct psim_ctx {
 
 namespace {
 
-void fill_constants(psim_ctx *c)
-{
-    const psim_params &P = c->prm;
-    Geo &g = c->geo;
-    g.n = P.n_particles;
-    g.S = P.cells_per_dim;
-    g.S2 = g.S * g.S;
-    g.ncell = g.S2 * g.S;
-    g.L = P.box;
-    g.half_pbc = P.box / 2.0;
-    if (P.half_uv > 0)
-        g.half_uv = P.half_uv;
-    else if (std::floor(P.box) == P.box)
-        g.half_uv = (double)((long long)P.box / 2);   // int L / 2 (utilities.cpp:124)
-    else
-        g.half_uv = P.box / 2.0;
-    g.cell = P.cell_size;
-    g.exact_mi = (g.S < 4) ? 1 : 0;
-    g.pad = 0;
-
-    Pot &p = c->pot;
-    p.mask = P.force_mask;
-    p.lj_num = P.lj_num;
-    const double rc = P.cutoff;
-    p.rc2 = rc * rc;
-    // guard band: the fast image distance differs from pbcDist's by a few
-    // ulps of L on each component
-    double band = 1e-12 * std::fmax(1.0, P.box / rc);
-    p.rc2_hi = p.rc2 * (1.0 + band);
-    p.rc2_lo = p.rc2 * (1.0 - band);
-    p.ov_factor = 0.8;
-    p.ov_band = band;
-    p.lj_neg_kt_eps = -P.kT * P.lj_well_depth;
-    p.lj_four_n = 4.0 * P.lj_num;
-    p.debye_len = P.debye_length;
-    p.debye_inv = 1.0 / P.debye_length;
-    p.yuk_a = P.yukawa_strength;
-    // AOPotential.cpp:50-55
-    double q = (rc / (rc - 1.0));
-    double a1 = -P.kT * P.ao_well_depth * q * q * q;
-    double a2 = -3.0 / (2.0 * rc);
-    double a3 = 1.0 / (2.0 * rc * rc * rc);
-    p.ao_c1 = -a1 * a2;
-    p.ao_c2 = -3.0 * a1 * a3;
-
-    // brownianIntegrator.cpp:53-81, 111-175 (host FP64, glibc libm)
-    const double kT = P.kT, gamma = P.gamma, dt = P.dt;
-    double y = gamma * dt;
-    double coeff0, coeff1, coeff2, coeff3, sig1, sig2, gn, goy2, goy3, hn;
-    {
-        double ty = 2.0 * y;
-        coeff0 = std::exp(-y);
-        double aa1 = 1.0 - coeff0;
-        double aa2 = 0.5 * y * (1.0 + coeff0) - aa1;
-        double aa3 = y - aa1;
-        coeff1 = aa1 / y;
-        coeff2 = aa2 / (y * y);
-        coeff3 = aa3 / (y * y);
-        sig1 = 2.0 * y - 3.0 + 4.0 * std::exp(-y) - std::exp(-ty);
-        sig2 = -2.0 * y - 3.0 + 4.0 * std::exp(y) - std::exp(ty);
-        gn = std::exp(y) - ty - std::exp(-y);
-        goy2 = gn / (y * y);
-        goy3 = gn / (y * y * y);
-        hn = y / (std::exp(y) - std::exp(-y));
-    }
-    if (gamma < 0.05) {
-        double y1 = y, y2 = y1 * y1, y3 = y2 * y1, y4 = y3 * y1, y5 = y4 * y1;
-        double y6 = y5 * y1, y7 = y6 * y1, y8 = y7 * y1, y9 = y8 * y1;
-        coeff1 = 1.0 - 0.5 * y1 + (1.0 / 6.0) * y2 - (1.0 / 24.0) * y3 + (1.0 / 120.0) * y4;
-        coeff2 = (1.0 / 12.0) * y1 - (1.0 / 24.0) * y2 + (1.0 / 80.0) * y3 - (1.0 / 360.0) * y4;
-        coeff3 = 0.5 - (1.0 / 6.0) * y1 + (1.0 / 24.0) * y2 - (1.0 / 120.0) * y3;
-        sig1 = +(2.0 / 3.0) * y3 - 0.5 * y4 + (7.0 / 30.0) * y5 - (1.0 / 12.0) * y6
-             + (31.0 / 1260.0) * y7 - (1.0 / 160.0) * y8 + (127.0 / 90720.0) * y9;
-        sig2 = -(2.0 / 3.0) * y3 - 0.5 * y4 - (7.0 / 30.) * y5 - (1.0 / 12.0) * y6
-             - (31.0 / 1260.0) * y7 - (1.0 / 160.0) * y8 - (127.0 / 90720.0) * y9;
-        goy2 = (1.0 / 3.0) * y1 + (1.0 / 60.0) * y3;
-        goy3 = 1.0 / 3.0 + (1.0 / 60.0) * y2;
-        hn = 0.5 - (1.00 / 12.0) * y2 + (7.0 / 720.0) * y4;
-        gn = (1.0 / 3.0) * y3 + (1.0 / 60.0) * y5;
-    }
-    if (gamma == 0) {
-        coeff0 = 1.0;
-        coeff1 = 1.0;
-        coeff2 = 0.0;
-        coeff3 = 0.5;
-    }
-    double gamma2 = gamma * gamma;
-    sig1 = std::sqrt(+kT * sig1 / gamma2);
-    sig2 = std::sqrt(-kT * sig2 / gamma2);
-    double corr = (kT / (gamma2)) * (gn / (sig1 * sig2));
-    double dev = std::sqrt(1.0 - (corr * corr));
-    GB &b = c->gb;
-    b.coeff0 = coeff0;
-    b.coeff1 = coeff1;
-    b.coeff3 = coeff3;
-    b.c0 = 1.0 + coeff0;
-    double m = 1.0 / P.mass;
-    double dt2 = dt * dt;
-    b.c1 = m * dt2 * coeff1;
-    b.c2 = m * dt2 * coeff2;
-    b.sig1 = sig1;
-    b.sig2 = sig2;
-    b.corr = corr;
-    b.dev = dev;
-    b.m = m;
-    b.dt = dt;
-    double dt3 = dt * dt2;
-    b.g2 = m * dt2 * goy2;
-    b.g3 = m * dt3 * goy3;
-    b.g4 = hn * (1.0 / dt);
-}
+void fill_constants(psim_ctx *c) { make_constants(c->prm, c->geo, c->pot, c->gb); }
 
 template <class T>
 int dalloc(T **p, size_t count)
@@ -699,6 +217,7 @@ int run_steps(psim_ctx *c, int nsteps, const double *d_noise)
         a.noise = d_noise ? d_noise + (size_t)s * 6 * n : nullptr;
         a.seed = c->prm.seed;
         a.step = c->step_index;
+        a.ctl = nullptr;
         a.first = (c->time == 0) ? 1 : 0;   // brownianIntegrator.cpp:183
         a.do_vel = 0;
         if (!a.first) {

@@ -1,0 +1,530 @@
+// psim_ensemble.hip -- many independent systems advanced together.
+//
+// A replica ensemble stacks R systems of uneven N, box, cell grid and force
+// strength in one set of arrays:
+//   particles: replica r owns original ids [p0_r, p0_r + n_r); the state is
+//     kept in cell order over the whole ensemble (SoA [3][Ntot]);
+//   cells: replica r owns global cells [c0_r, c0_r + S_r^3), so one counting
+//     sort over the global cell keys rebuilds every replica's table at once
+//     and the replicas stay contiguous in slot order.
+// Each step is one launch per kernel stage for all replicas (pair forces +
+// G&B, scan, scatter, reorder, control), so a step of 512 replicas costs the
+// same number of launches as one system, and the sequence can be captured
+// once in a hipGraph and replayed.  The step counter, time and velocity-step
+// flags live in device memory (StepCtl) so replays advance them.
+//
+// The pair kernel is the thread-per-particle gather (these systems hold
+// ~0.2-1.5 particles per cell); its workgroup size is chosen by a measured
+// autotune over the candidates in kEnsBlocks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "psim_build.hpp"
+#include "psim_gpu.h"
+#include "psim_params.hpp"
+#include "psim_step.hpp"
+
+using namespace psim;
+
+namespace {
+
+struct RepGeo {
+    Geo g;       // g.n = Ntot (array stride), g.c0 = first cell of the replica
+    Pot p;
+    double radius;
+};
+
+// replica owning original id o: last r with p0[r] <= o
+__device__ __forceinline__ int replica_of(const int *__restrict__ p0, int nrep, int o)
+{
+    int lo = 0, hi = nrep - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (p0[mid] <= o) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int MASK, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_ens_step(const RepGeo *__restrict__ reps,
+                                                    const int *__restrict__ p0, int nrep,
+                                                    int ntot, GB gb, StepArgs a)
+{
+    __shared__ unsigned hits[kHitCap * BLOCK];
+    const int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= ntot) return;
+    const int r = replica_of(p0, nrep, a.id[s]);
+    const Geo g = reps[r].g;
+    const Pot p = reps[r].p;
+    const double radius = reps[r].radius;
+    const int N = ntot;
+    const double xi = a.pos[s], yi = a.pos[N + s], zi = a.pos[2 * N + s];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    SoAPos src{a.pos, N, radius};
+    StartTab tab{a.cstart};
+    Orig orig{a.id, 1, 0};
+    if (g.exact_mi)
+        gather_force<MASK, true, BLOCK>(g, p, src, tab, s, xi, yi, zi, radius,
+                                        hits + threadIdx.x, fx, fy, fz, a.err, orig);
+    else
+        gather_force<MASK, false, BLOCK>(g, p, src, tab, s, xi, yi, zi, radius,
+                                         hits + threadIdx.x, fx, fy, fz, a.err, orig);
+    integrate_one<true>(g, gb, a, s, fx, fy, fz);
+}
+
+// Initial hash of every particle into its replica's cells.
+__global__ void k_ens_hash(const RepGeo *__restrict__ reps, const int *__restrict__ p0,
+                           int nrep, int ntot, const double *pos, const int *id, int *key,
+                           int *arank, int *count)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ntot) return;
+    const Geo g = reps[replica_of(p0, nrep, id[s])].g;
+    const int k = g.c0 + cell_1d(pos[s], g.cell, g.S) + g.S * cell_1d(pos[ntot + s], g.cell, g.S)
+                + g.S2 * cell_1d(pos[2 * ntot + s], g.cell, g.S);
+    key[s] = k;
+    arank[s] = wave_aggregated_rank(count, k);
+}
+
+// End of step: advance the clock and set the flags of the next step
+// (system.cpp:373; brownianIntegrator.cpp:183, 288-292).
+__global__ void k_ctl_advance(StepCtl *c)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->step += 1;
+        c->time += c->dt;
+        ctl_prepare(c);
+    }
+}
+
+constexpr int kEnsBlocks[3] = {64, 128, 256};
+
+}  // namespace
+
+struct psim_ens {
+    psim_params common;
+    GB gb;
+    int nrep = 0, ntot = 0, ncell = 0, nscan = 0;
+    int mask = 0;
+    int block = 256;
+    hipStream_t stream = nullptr;
+    RepGeo *reps = nullptr;
+    int *p0 = nullptr;
+    std::vector<int> h_p0, h_c0;
+    std::vector<double> h_box;
+    double *pos = nullptr, *pos0 = nullptr, *frc = nullptr, *mem = nullptr;
+    double *npos = nullptr, *npos0 = nullptr, *nfrc = nullptr, *nmem = nullptr;
+    double *vel = nullptr;
+    int *id = nullptr, *nid = nullptr, *key = nullptr, *arank = nullptr, *perm = nullptr;
+    int *count = nullptr, *cstart = nullptr, *bsum = nullptr;
+    DevErr *err = nullptr;
+    StepCtl *ctl = nullptr;
+    bool have_state = false;
+    int graph_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+#define ECK(x)                                                                   \
+    do {                                                                         \
+        hipError_t _e = (x);                                                     \
+        if (_e != hipSuccess) {                                                  \
+            std::fprintf(stderr, "psim_ens: %s failed: %s (%s:%d)\n", #x,        \
+                         hipGetErrorString(_e), __FILE__, __LINE__);             \
+            return PSIM_E_HIP;                                                   \
+        }                                                                        \
+    } while (0)
+
+namespace {
+
+template <class T>
+int ealloc(T **p, size_t count)
+{
+    if (count == 0) count = 1;
+    if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return PSIM_E_NOMEM;
+    }
+    return 0;
+}
+
+inline unsigned eblocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+StepArgs ens_args(psim_ens *e)
+{
+    StepArgs a{};
+    a.pos = e->pos; a.pos0 = e->pos0; a.frc = e->frc; a.mem = e->mem;
+    a.id = e->id; a.cstart = e->cstart;
+    a.npos = e->npos; a.npos0 = e->npos0; a.nfrc = e->nfrc; a.nmem = e->nmem;
+    a.nid = e->nid; a.vel = e->vel; a.key = e->key; a.arank = e->arank;
+    a.count = e->count; a.err = e->err;
+    a.noise = nullptr;
+    a.seed = e->common.seed;
+    a.ctl = e->ctl;
+    return a;
+}
+
+template <int MASK, int BLOCK>
+void launch_ens_pair(psim_ens *e, const StepArgs &a)
+{
+    k_ens_step<MASK, BLOCK><<<eblocks(e->ntot, BLOCK), BLOCK, 0, e->stream>>>(
+        e->reps, e->p0, e->nrep, e->ntot, e->gb, a);
+}
+
+template <int MASK>
+void launch_ens_pair_block(psim_ens *e, const StepArgs &a)
+{
+    switch (e->block) {
+    case 64: launch_ens_pair<MASK, 64>(e, a); break;
+    case 128: launch_ens_pair<MASK, 128>(e, a); break;
+    default: launch_ens_pair<MASK, 256>(e, a); break;
+    }
+}
+
+void launch_build_ens(psim_ens *e)
+{
+    const int n = e->ntot, nc = e->ncell;
+    hipStream_t st = e->stream;
+    k_scan_reduce<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->bsum);
+    k_scan_bsums<<<1, kScanThreads, 0, st>>>(e->bsum, e->nscan);
+    k_scan_apply<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->bsum, e->cstart, n);
+    k_scatter<<<eblocks(n, 256), 256, 0, st>>>(n, e->key, e->arank, e->cstart, e->perm);
+    k_move<<<eblocks(n, 256), 256, 0, st>>>(n, e->perm, e->key, e->cstart, e->npos, e->npos0,
+                                            e->nfrc, e->nmem, e->nid, e->pos, e->pos0, e->frc,
+                                            e->mem, e->id);
+}
+
+// One full step, enqueued on the ensemble's stream (also used while
+// capturing the graph).
+int enqueue_step(psim_ens *e)
+{
+    StepArgs a = ens_args(e);
+    switch (e->mask) {
+    case 1: launch_ens_pair_block<1>(e, a); break;
+    case 2: launch_ens_pair_block<2>(e, a); break;
+    case 3: launch_ens_pair_block<3>(e, a); break;
+    case 4: launch_ens_pair_block<4>(e, a); break;
+    case 5: launch_ens_pair_block<5>(e, a); break;
+    case 6: launch_ens_pair_block<6>(e, a); break;
+    case 7: launch_ens_pair_block<7>(e, a); break;
+    default: return PSIM_E_ARG;
+    }
+    launch_build_ens(e);
+    k_ctl_advance<<<1, 64, 0, e->stream>>>(e->ctl);
+    ECK(hipGetLastError());
+    return 0;
+}
+
+void drop_graph(psim_ens *e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+int capture_graph(psim_ens *e)
+{
+    drop_graph(e);
+    ECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int k = 0; k < e->graph_steps && !rc; ++k) rc = enqueue_step(e);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ce != hipSuccess) {
+        std::fprintf(stderr, "psim_ens: graph capture failed: %s\n", hipGetErrorString(ce));
+        return PSIM_E_HIP;
+    }
+    e->graph = g;
+    ECK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psim_ens_create(const psim_params *common, const psim_replica *reps, int nrep,
+                    psim_ens **out)
+{
+    if (!common || !reps || nrep <= 0 || !out) return PSIM_E_ARG;
+    *out = nullptr;
+    const psim_params &C = *common;
+    if (C.force_mask <= 0 || C.force_mask > 7 || !(C.dt > 0) || !(C.mass > 0)) return PSIM_E_ARG;
+    psim_ens *e = new (std::nothrow) psim_ens();
+    if (!e) return PSIM_E_NOMEM;
+    e->common = C;
+    e->mask = C.force_mask;
+    e->nrep = nrep;
+    std::vector<RepGeo> h(nrep);
+    long long ntot = 0, ncell = 0;
+    e->h_p0.resize(nrep + 1);
+    e->h_c0.resize(nrep + 1);
+    e->h_box.resize(nrep);
+    for (int r = 0; r < nrep; ++r) {
+        const psim_replica &R = reps[r];
+        if (R.n_particles <= 0 || R.cells_per_dim < 3 || R.cells_per_dim > 1290 ||
+            !(R.box > 0) || !(R.cell_size > 0) || !(R.cutoff > 0)) {
+            delete e;
+            return PSIM_E_ARG;
+        }
+        psim_params P = C;
+        P.n_particles = R.n_particles;
+        P.cells_per_dim = R.cells_per_dim;
+        P.box = R.box;
+        P.cell_size = R.cell_size;
+        P.half_uv = 0.0;
+        P.cutoff = R.cutoff;
+        P.lj_num = R.lj_num;
+        P.lj_well_depth = R.lj_well_depth;
+        P.yukawa_strength = R.yukawa_strength;
+        P.debye_length = R.debye_length > 0 ? R.debye_length : 0.5;
+        P.ao_well_depth = R.ao_well_depth;
+        Geo g;
+        Pot p;
+        GB b;
+        make_constants(P, g, p, b);
+        if (r == 0) e->gb = b;
+        g.c0 = (int)ncell;
+        h[r].g = g;
+        h[r].p = p;
+        h[r].radius = C.radius;
+        e->h_p0[r] = (int)ntot;
+        e->h_c0[r] = (int)ncell;
+        e->h_box[r] = R.box;
+        ntot += R.n_particles;
+        ncell += (long long)g.ncell;
+    }
+    e->h_p0[nrep] = (int)ntot;
+    e->h_c0[nrep] = (int)ncell;
+    if (ntot >= (1 << kIdxBits) || ncell >= (1ll << 31) - 1) {
+        delete e;
+        return PSIM_E_UNSUPPORTED;
+    }
+    e->ntot = (int)ntot;
+    e->ncell = (int)ncell;
+    for (auto &rg : h) rg.g.n = e->ntot;   // SoA stride of the stacked arrays
+    e->nscan = (e->ncell + kScanTile - 1) / kScanTile;
+    const size_t n3 = 3 * (size_t)e->ntot;
+    int rc = 0;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) rc = PSIM_E_HIP;
+    double **dbufs[] = {&e->pos, &e->pos0, &e->frc, &e->mem, &e->npos, &e->npos0, &e->nfrc,
+                        &e->nmem, &e->vel};
+    for (double **b : dbufs)
+        if (!rc) rc = ealloc(b, n3);
+    int **ibufs[] = {&e->id, &e->nid, &e->key, &e->arank, &e->perm};
+    for (int **b : ibufs)
+        if (!rc) rc = ealloc(b, (size_t)e->ntot);
+    if (!rc) rc = ealloc(&e->count, (size_t)e->ncell);
+    if (!rc) rc = ealloc(&e->cstart, (size_t)e->ncell + 1);
+    if (!rc) rc = ealloc(&e->bsum, (size_t)e->nscan);
+    if (!rc) rc = ealloc(&e->err, 1);
+    if (!rc) rc = ealloc(&e->ctl, 1);
+    if (!rc) rc = ealloc(&e->reps, (size_t)nrep);
+    if (!rc) rc = ealloc(&e->p0, (size_t)nrep + 1);
+    if (!rc && hipMemcpy(e->reps, h.data(), sizeof(RepGeo) * nrep, hipMemcpyHostToDevice) != hipSuccess)
+        rc = PSIM_E_HIP;
+    if (!rc && hipMemcpy(e->p0, e->h_p0.data(), sizeof(int) * (nrep + 1), hipMemcpyHostToDevice) != hipSuccess)
+        rc = PSIM_E_HIP;
+    if (!rc && hipMemset(e->count, 0, sizeof(int) * e->ncell) != hipSuccess) rc = PSIM_E_HIP;
+    if (!rc && hipMemset(e->err, 0, sizeof(DevErr)) != hipSuccess) rc = PSIM_E_HIP;
+    if (rc) {
+        psim_ens_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return 0;
+}
+
+void psim_ens_destroy(psim_ens *e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    void *bufs[] = {e->pos, e->pos0, e->frc, e->mem, e->npos, e->npos0, e->nfrc, e->nmem,
+                    e->vel, e->id, e->nid, e->key, e->arank, e->perm, e->count, e->cstart,
+                    e->bsum, e->err, e->ctl, e->reps, e->p0};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int psim_ens_total_particles(psim_ens *e) { return e ? e->ntot : PSIM_E_ARG; }
+
+int psim_ens_set_state(psim_ens *e, const double *pos, const double *vel, double current_time)
+{
+    if (!e || !pos) return PSIM_E_ARG;
+    const int n = e->ntot;
+    std::vector<double> h(3 * (size_t)n);
+    auto up = [&](const double *src, double *dst, bool pos0) -> int {
+        int r = 0;
+        for (int i = 0; i < n; ++i) {
+            while (i >= e->h_p0[r + 1]) ++r;
+            const double L = e->h_box[r];
+            for (int d = 0; d < 3; ++d) {
+                double v = 0.0;
+                if (src) {
+                    v = src[3 * (size_t)i + d];
+                } else if (pos0) {
+                    // a fresh particle: pos0 = safeMod0(0, x) (particle.cpp:31-66)
+                    const double x = pos[3 * (size_t)i + d];
+                    v = (std::fabs(x) > (L / 2)) ? ((x < 0) ? -L : L) : 0.0;
+                }
+                h[(size_t)d * n + i] = v;
+            }
+        }
+        ECK(hipMemcpy(dst, h.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        return 0;
+    };
+    int rc;
+    if ((rc = up(pos, e->npos, false))) return rc;
+    if ((rc = up(nullptr, e->npos0, true))) return rc;
+    if ((rc = up(vel, e->vel, false))) return rc;
+    if ((rc = up(nullptr, e->nfrc, false))) return rc;
+    if ((rc = up(nullptr, e->nmem, false))) return rc;
+    std::vector<int> ids(n);
+    for (int i = 0; i < n; ++i) ids[i] = i;
+    ECK(hipMemcpy(e->nid, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    ECK(hipMemset(e->count, 0, sizeof(int) * e->ncell));
+    StepCtl c{};
+    c.step = 0;
+    c.time = current_time;
+    c.dt = e->common.dt;
+    c.vel_counter = 0;
+    c.vel_freq = e->common.vel_freq;
+    c.first = (current_time == 0) ? 1 : 0;
+    c.do_vel = 0;
+    if (!c.first) {
+        // as the host-driven engine does at the top of a step
+        c.do_vel = (c.vel_freq == 0 || c.vel_counter == c.vel_freq) ? 1 : 0;
+        c.vel_counter = (c.vel_counter == c.vel_freq) ? 0 : c.vel_counter + 1;
+    }
+    ECK(hipMemcpy(e->ctl, &c, sizeof(c), hipMemcpyHostToDevice));
+    k_ens_hash<<<eblocks(n, 256), 256, 0, e->stream>>>(e->reps, e->p0, e->nrep, n, e->npos,
+                                                       e->nid, e->key, e->arank, e->count);
+    launch_build_ens(e);
+    ECK(hipGetLastError());
+    ECK(hipStreamSynchronize(e->stream));
+    e->have_state = true;
+    return 0;
+}
+
+int psim_ens_get_state(psim_ens *e, double *pos, double *pos0, double *vel, double *frc)
+{
+    if (!e || !e->have_state) return PSIM_E_ARG;
+    const int n = e->ntot;
+    std::vector<int> ids(n);
+    std::vector<double> h(3 * (size_t)n);
+    ECK(hipStreamSynchronize(e->stream));
+    ECK(hipMemcpy(ids.data(), e->id, sizeof(int) * n, hipMemcpyDeviceToHost));
+    auto down = [&](const double *src, double *dst, bool by_slot) -> int {
+        if (!dst) return 0;
+        ECK(hipMemcpy(h.data(), src, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        for (int s = 0; s < n; ++s) {
+            const int o = by_slot ? ids[s] : s;
+            for (int d = 0; d < 3; ++d) dst[3 * (size_t)o + d] = h[(size_t)d * n + s];
+        }
+        return 0;
+    };
+    int rc;
+    if ((rc = down(e->pos, pos, true))) return rc;
+    if ((rc = down(e->pos0, pos0, true))) return rc;
+    if ((rc = down(e->vel, vel, false))) return rc;
+    if ((rc = down(e->frc, frc, true))) return rc;
+    return 0;
+}
+
+int psim_ens_set_graph(psim_ens *e, int steps_per_graph)
+{
+    if (!e || steps_per_graph < 0) return PSIM_E_ARG;
+    drop_graph(e);
+    e->graph_steps = steps_per_graph;
+    return 0;
+}
+
+int psim_ens_step(psim_ens *e, int nsteps)
+{
+    if (!e || nsteps < 0 || !e->have_state) return PSIM_E_ARG;
+    int done = 0;
+    if (e->graph_steps > 0) {
+        if (!e->exec) {
+            int rc = capture_graph(e);
+            if (rc) return rc;
+        }
+        for (; done + e->graph_steps <= nsteps; done += e->graph_steps)
+            ECK(hipGraphLaunch(e->exec, e->stream));
+    }
+    for (; done < nsteps; ++done) {
+        int rc = enqueue_step(e);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int psim_ens_sync(psim_ens *e)
+{
+    if (!e) return PSIM_E_ARG;
+    ECK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int psim_ens_last_error(psim_ens *e, int *code, int *i, int *j)
+{
+    if (!e) return PSIM_E_ARG;
+    DevErr h{};
+    ECK(hipStreamSynchronize(e->stream));
+    ECK(hipMemcpy(&h, e->err, sizeof(h), hipMemcpyDeviceToHost));
+    if (code) *code = h.code;
+    if (i) *i = h.i;
+    if (j) *j = h.j;
+    return 0;
+}
+
+// Time every candidate workgroup size for trial_steps steps (after one
+// warm-up step each) and keep the fastest.  The steps taken while tuning
+// advance the ensemble like any other steps.
+int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_block, float *best_ms)
+{
+    if (!e || trial_steps <= 0 || !e->have_state) return PSIM_E_ARG;
+    hipEvent_t t0, t1;
+    ECK(hipEventCreate(&t0));
+    ECK(hipEventCreate(&t1));
+    float best = 0.f;
+    int best_block = e->block;
+    for (int b : kEnsBlocks) {
+        e->block = b;
+        int rc = enqueue_step(e);
+        if (rc) return rc;
+        ECK(hipEventRecord(t0, e->stream));
+        for (int k = 0; k < trial_steps; ++k)
+            if ((rc = enqueue_step(e))) return rc;
+        ECK(hipEventRecord(t1, e->stream));
+        ECK(hipEventSynchronize(t1));
+        float ms = 0.f;
+        ECK(hipEventElapsedTime(&ms, t0, t1));
+        if (best_block == 0 || best == 0.f || ms < best) {
+            best = ms;
+            best_block = b;
+        }
+    }
+    e->block = best_block;
+    drop_graph(e);   // recaptured with the chosen kernel on the next step
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (chosen_block) *chosen_block = best_block;
+    if (best_ms) *best_ms = best / trial_steps;
+    return 0;
+}
+
+}  // extern "C"

@@ -78,8 +78,9 @@ __device__ inline double shifted(double xi, int k, double L)
     return k == 0 ? xi : (k == 1 ? xi + L : xi - L);
 }
 
-// Drain one lane's hit list: pair forces for entries [0, nh).
-template <int MASK, bool EXACT, class Src>
+// Drain one lane's hit list: pair forces for entries [0, nh).  The list is a
+// column of stride BLOCK in the block's LDS array.
+template <int MASK, bool EXACT, int BLOCK = kBlock, class Src>
 __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
                            double xi, double yi, double zi, double ri,
                            const unsigned *hits, int nh, double &fx,
@@ -87,7 +88,7 @@ __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
                            const Orig &orig)
 {
     for (int k = 0; k < nh; ++k) {
-        unsigned e = hits[k * kBlock];
+        unsigned e = hits[k * BLOCK];
         int j = (int)(e & kIdxMask);
         double xj = src.x(j), yj = src.y(j), zj = src.z(j);
         double size = ri + src.rad(j);
@@ -131,7 +132,7 @@ __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
 
 // Force on the particle in slot s (position xi,yi,zi, radius ri).
 // hits points at this lane's column of the block's LDS hit list.
-template <int MASK, bool EXACT, class Src, class Tab>
+template <int MASK, bool EXACT, int BLOCK = kBlock, class Src, class Tab>
 __device__ void gather_force(const Geo &g, const Pot &p, const Src &src,
                              const Tab &tab, int s, double xi, double yi,
                              double zi, double ri, unsigned *hits, double &fx,
@@ -155,7 +156,7 @@ __device__ void gather_force(const Geo &g, const Pot &p, const Src &src,
                 int rz = cz + oz, kz = 0;
                 if (rz < 0) { rz += S; kz = 1; } else if (rz >= S) { rz -= S; kz = 2; }
                 double zs = shifted(zi, kz, g.L);
-                int h = rx + S * ry + g.S2 * rz;
+                int h = g.c0 + rx + S * ry + g.S2 * rz;
                 int b, e;
                 tab.range(h, b, e);
                 unsigned code = (unsigned)(kx + 3 * ky + 9 * kz) << kIdxBits;
@@ -173,20 +174,20 @@ __device__ void gather_force(const Geo &g, const Pot &p, const Src &src,
                         hit = (r2 < p.rc2) && (j != s);
                     }
                     if (hit) {
-                        hits[nh * kBlock] = (unsigned)j | code;
+                        hits[nh * BLOCK] = (unsigned)j | code;
                         ++nh;
                     }
                     if (__any(nh == kHitCap)) {
-                        drain_hits<MASK, EXACT>(g, p, src, s, xi, yi, zi, ri, hits,
-                                                nh, fx, fy, fz, err, orig);
+                        drain_hits<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, ri, hits,
+                                                       nh, fx, fy, fz, err, orig);
                         nh = 0;
                     }
                 }
             }
         }
     }
-    drain_hits<MASK, EXACT>(g, p, src, s, xi, yi, zi, ri, hits, nh, fx, fy, fz,
-                            err, orig);
+    drain_hits<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, ri, hits, nh, fx, fy, fz,
+                                   err, orig);
 }
 
 }  // namespace psim

@@ -1,0 +1,279 @@
+// psim_step.hpp -- the step kernels of one system: pair forces (both
+// kernel paths), the G&B integrator epilogue and force-only evaluations.
+#pragma once
+
+#include "psim_dense.hpp"
+#include "psim_pairs.hpp"
+
+namespace psim {
+namespace {
+
+// mean particles per cell above which the cell-per-workgroup kernel is used
+constexpr double kDenseMinOccupancy = 2.0;
+
+// Step control kept in device memory so that a captured sequence of steps
+// can be replayed: the step kernels read it, the reorder kernel advances it.
+// (brownianIntegrator.cpp:183 first-step test, :288-292 velocity counter,
+// system.cpp:373 time advance)
+struct StepCtl {
+    unsigned long long step;   // Philox counter of the current step
+    double time;               // currentTime at the start of the step
+    double dt;
+    int vel_counter;
+    int vel_freq;
+    int first;                 // currentTime == 0
+    int do_vel;                // velocityStep this step
+};
+
+__device__ __forceinline__ void ctl_prepare(StepCtl *c)
+{
+    c->first = (c->time == 0) ? 1 : 0;
+    c->do_vel = 0;
+    if (!c->first) {
+        c->do_vel = (c->vel_freq == 0 || c->vel_counter == c->vel_freq) ? 1 : 0;
+        if (c->vel_counter == c->vel_freq) c->vel_counter = 0; else c->vel_counter++;
+    }
+}
+
+struct StepArgs {
+    const double *pos, *pos0, *frc, *mem;
+    const int *id, *cstart;
+    double *npos, *npos0, *nfrc, *nmem;
+    int *nid;
+    double *vel;         // [3][N] by original index
+    int *key, *arank, *count;
+    DevErr *err;
+    const double *noise; // [6][N] by original index, or null -> Philox
+    unsigned long long seed, step;
+    int first, do_vel;
+    const StepCtl *ctl;  // when set, step / first / do_vel are read from it
+};
+
+// G&B epilogue for the particle in slot s with total force F
+// (brownianIntegrator.cpp:191-356, particle.cpp:61-99); writes the next state
+// at slot s, the new cell key and an atomic per-cell rank.
+template <bool WRITE_F>
+__device__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int s,
+                              double fx, double fy, double fz)
+{
+    const int N = g.n;
+    const int o = a.id[s];
+    const int first = a.ctl ? a.ctl->first : a.first;
+    const int do_vel = a.ctl ? a.ctl->do_vel : a.do_vel;
+    double z[6];
+    if (a.noise) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) z[k] = a.noise[(size_t)k * N + o];
+    } else {
+        philox_normals6(a.seed, a.ctl ? a.ctl->step : a.step, (uint32_t)o, z);
+    }
+    const double xc[3] = {a.pos[s], a.pos[N + s], a.pos[2 * N + s]};
+    const double F[3] = {fx, fy, fz};
+    double x0c[3], xn[3], M[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x0c[d] = a.pos0[d * N + s];
+    if (first) {
+        // firstStep :200-218
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            M[d] = z[3 + d];
+            double v = a.vel[(size_t)d * N + o];
+            xn[d] = xc[d] + (v * gb.coeff1 * gb.dt)
+                  + (F[d] * gb.coeff3 * gb.dt * gb.dt * gb.m) + (gb.sig1 * M[d]);
+        }
+    } else {
+        // normalStep :238-274
+        double Mo[3], Fo[3], Cr[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            Mo[d] = a.mem[d * N + s];
+            Fo[d] = a.frc[d * N + s];
+            Cr[d] = gb.sig2 * ((gb.corr * Mo[d]) + (gb.dev * z[d]));
+            M[d] = z[3 + d];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double q = (gb.c0 * xc[d]);
+            q -= (gb.coeff0 * x0c[d]);
+            q += (gb.c1 * F[d]);
+            q += (gb.c2 * (F[d] - Fo[d]));
+            q += (gb.sig1 * M[d]) + (gb.coeff0 * Cr[d]);
+            xn[d] = q;
+        }
+        if (do_vel) {
+            // velocityStep :305-354
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double dx0 = xc[d] - x0c[d];
+                double xN = safe_mod(xn[d], g.L);
+                double x0 = safe_mod0(xc[d], xN, g.L);
+                double dx = xN - x0;
+                double vv = dx + dx0;
+                vv += (gb.g2 * F[d]);
+                vv -= (gb.g3 * (F[d] - Fo[d]));
+                vv += (Cr[d] - gb.sig1 * M[d]);
+                vv *= gb.g4;
+                a.vel[(size_t)d * N + o] = vv;
+            }
+        }
+    }
+    bool bad = false;
+    double xw[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        xw[d] = safe_mod(xn[d], g.L);
+        double x0n = safe_mod0(xc[d], xw[d], g.L);
+        bad |= (xw[d] < 0.0) || (xw[d] >= g.L);
+        a.npos[d * N + s] = xw[d];
+        a.npos0[d * N + s] = x0n;
+        if (WRITE_F) a.nfrc[d * N + s] = F[d];
+        a.nmem[d * N + s] = M[d];
+    }
+    a.nid[s] = o;
+    if (bad) report_error(a.err, 7703, o, -1);
+    int k = g.c0 + cell_1d(xw[0], g.cell, g.S) + g.S * cell_1d(xw[1], g.cell, g.S)
+          + g.S2 * cell_1d(xw[2], g.cell, g.S);
+    a.key[s] = k;
+    a.arank[s] = wave_aggregated_rank(a.count, k);
+}
+
+// Thread-per-particle step (sparse systems and S < 4).
+template <int MASK>
+__global__ __launch_bounds__(kBlock) void k_step(Geo g, Pot p, GB gb, double radius,
+                                                 StepArgs a)
+{
+    __shared__ unsigned hits[kHitCap * kBlock];
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= g.n) return;
+    const int N = g.n;
+    const double xi = a.pos[s], yi = a.pos[N + s], zi = a.pos[2 * N + s];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    SoAPos src{a.pos, N, radius};
+    StartTab tab{a.cstart};
+    Orig orig{a.id, 1, 0};
+    if (g.exact_mi)
+        gather_force<MASK, true>(g, p, src, tab, s, xi, yi, zi, radius,
+                                 hits + threadIdx.x, fx, fy, fz, a.err, orig);
+    else
+        gather_force<MASK, false>(g, p, src, tab, s, xi, yi, zi, radius,
+                                  hits + threadIdx.x, fx, fy, fz, a.err, orig);
+    integrate_one<true>(g, gb, a, s, fx, fy, fz);
+}
+
+// Cell-per-workgroup forces for the step (dense systems, S >= 4): the total
+// force on each particle goes to nfrc[slot], where k_integrate picks it up.
+// (Fusing the G&B epilogue here would run it on n_i of the workgroup's lanes,
+// latency-bound; the separate kernel runs it at full occupancy.)
+template <int MASK>
+__global__ __launch_bounds__(kDenseThreads) void k_step_dense(Geo g, Pot p, double radius,
+                                                              const double *pos,
+                                                              const int *id,
+                                                              const int *cstart,
+                                                              double *nfrc, DevErr *err,
+                                                              int per_xcd)
+{
+    __shared__ DenseShared sh;
+    const int c = xcd_cell(blockIdx.x, per_xcd);
+    if (c >= g.ncell) return;
+    const int home0 = cstart[c];
+    const int N = g.n;
+    dense_cell_forces<MASK>(g, p, radius, pos, cstart, id, err, c, sh,
+                            [=](int i, double Fx, double Fy, double Fz) {
+                                const int s = home0 + i;
+                                nfrc[s] = Fx;
+                                nfrc[N + s] = Fy;
+                                nfrc[2 * N + s] = Fz;
+                            });
+}
+
+// G&B step for every slot with the force already in nfrc (dense path).
+__global__ __launch_bounds__(256) void k_integrate(Geo g, GB gb, StepArgs a)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= g.n) return;
+    const int N = g.n;
+    integrate_one<false>(g, gb, a, s, a.nfrc[s], a.nfrc[N + s], a.nfrc[2 * N + s]);
+}
+
+// Forces only at the current state, written by original index ([3][N]).
+template <int MASK>
+__global__ __launch_bounds__(kBlock) void k_forces(Geo g, Pot p, double radius,
+                                                   const double *pos, const int *id,
+                                                   const int *cstart, double *fout,
+                                                   DevErr *err)
+{
+    __shared__ unsigned hits[kHitCap * kBlock];
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= g.n) return;
+    const int N = g.n;
+    const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    SoAPos src{pos, N, radius};
+    StartTab tab{cstart};
+    Orig orig{id, 1, 0};
+    if (g.exact_mi)
+        gather_force<MASK, true>(g, p, src, tab, s, xi, yi, zi, radius,
+                                 hits + threadIdx.x, fx, fy, fz, err, orig);
+    else
+        gather_force<MASK, false>(g, p, src, tab, s, xi, yi, zi, radius,
+                                  hits + threadIdx.x, fx, fy, fz, err, orig);
+    const int o = id[s];
+    fout[o] = fx;
+    fout[N + o] = fy;
+    fout[2 * N + o] = fz;
+}
+
+template <int MASK>
+__global__ __launch_bounds__(kDenseThreads) void k_forces_dense(Geo g, Pot p, double radius,
+                                                                const double *pos,
+                                                                const int *id,
+                                                                const int *cstart,
+                                                                double *fout, DevErr *err,
+                                                                int per_xcd)
+{
+    __shared__ DenseShared sh;
+    const int c = xcd_cell(blockIdx.x, per_xcd);
+    if (c >= g.ncell) return;
+    const int home0 = cstart[c];
+    const int N = g.n;
+    dense_cell_forces<MASK>(g, p, radius, pos, cstart, id, err, c, sh,
+                            [=](int i, double Fx, double Fy, double Fz) {
+                                const int o = id[home0 + i];
+                                fout[o] = Fx;
+                                fout[N + o] = Fy;
+                                fout[2 * N + o] = Fz;
+                            });
+}
+
+// Forces from the reference's host arrays (IForce::getAcceleration over the
+// whole system): sorted4, particleHashIndex (hash, orig), cellStartEnd.
+template <int MASK>
+__global__ __launch_bounds__(kBlock) void k_forces_table(Geo g, Pot p,
+                                                         const double *sorted4,
+                                                         const int *hash_index,
+                                                         const int *cse, double *fout,
+                                                         DevErr *err)
+{
+    __shared__ unsigned hits[kHitCap * kBlock];
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= g.n) return;
+    AoS4Pos src{sorted4};
+    PairTab tab{cse};
+    Orig orig{hash_index, 2, 1};
+    const double xi = src.x(s), yi = src.y(s), zi = src.z(s), ri = src.rad(s);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (g.exact_mi)
+        gather_force<MASK, true>(g, p, src, tab, s, xi, yi, zi, ri,
+                                 hits + threadIdx.x, fx, fy, fz, err, orig);
+    else
+        gather_force<MASK, false>(g, p, src, tab, s, xi, yi, zi, ri,
+                                  hits + threadIdx.x, fx, fy, fz, err, orig);
+    const int o = orig(s);
+    fout[3 * (size_t)o] = fx;   // particleForce[3*orig + k] (LJPotential.cpp:179-181)
+    fout[3 * (size_t)o + 1] = fy;
+    fout[3 * (size_t)o + 2] = fz;
+}
+
+
+}  // namespace
+}  // namespace psim

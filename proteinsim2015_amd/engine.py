@@ -394,3 +394,107 @@ class GpuSystem:
     @property
     def stream(self) -> int:
         return self.lib.psim_gpu_stream(self._ctx) or 0
+
+
+@dataclass
+class Replica:
+    """One member of a replica ensemble: its geometry and force strengths
+    (the integrator and the force modules are shared)."""
+    geometry: SystemGeometry
+    forces: object
+
+
+class Ensemble:
+    """Independent systems of uneven N, box and force strength advanced
+    together on the GPU (one launch per kernel stage for all replicas,
+    optionally replayed from a captured hipGraph)."""
+
+    def __init__(self, replicas, integrator: BrownianIntegrator, radius: float = 0.5):
+        self.lib = _lib.load()
+        if not replicas:
+            raise ValueError("empty ensemble")
+        sets = [r.forces if isinstance(r.forces, ForceSet) else ForceSet(
+            list(r.forces) if isinstance(r.forces, (list, tuple)) else [r.forces])
+            for r in replicas]
+        masks = {fs.mask for fs in sets}
+        if len(masks) != 1:
+            raise ValueError("all replicas must use the same force modules")
+        common = PsimParams()
+        common.force_mask = masks.pop()
+        common.radius = radius
+        common.mass, common.kT, common.gamma, common.dt = (integrator.mass, integrator.kT,
+                                                           integrator.gamma, integrator.dt)
+        common.vel_freq = integrator.vel_freq
+        common.seed = integrator.seed
+        arr = (_lib.PsimReplica * len(replicas))()
+        for k, (rep, fs) in enumerate(zip(replicas, sets)):
+            tmp = PsimParams()
+            fs.fill(tmp)
+            g = rep.geometry
+            a = arr[k]
+            a.n_particles, a.cells_per_dim, a.box, a.cell_size = g.n, g.cells_per_dim, g.box, \
+                g.cell_size
+            a.cutoff, a.lj_well_depth, a.yukawa_strength = tmp.cutoff, tmp.lj_well_depth, \
+                tmp.yukawa_strength
+            a.debye_length, a.ao_well_depth, a.lj_num = tmp.debye_length, tmp.ao_well_depth, \
+                tmp.lj_num
+        self.replicas = replicas
+        self.offsets = np.concatenate([[0], np.cumsum([r.geometry.n for r in replicas])])
+        self._ctx = C.c_void_p()
+        check(self.lib.psim_ens_create(C.byref(common), arr, len(replicas), C.byref(self._ctx)),
+              "psim_ens_create")
+        self.n = int(self.offsets[-1])
+
+    def close(self):
+        if self._ctx:
+            self.lib.psim_ens_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_state(self, pos, vel=None, time=0.0):
+        pos = _aos3(pos, self.n, "pos")
+        vel = _aos3(vel, self.n, "vel")
+        check(self.lib.psim_ens_set_state(self._ctx, _ptr(pos), _ptr(vel), float(time)),
+              "psim_ens_set_state")
+
+    def get_state(self) -> dict:
+        out = {k: np.empty(3 * self.n) for k in ("pos", "pos0", "vel", "frc")}
+        check(self.lib.psim_ens_get_state(self._ctx, _ptr(out["pos"]), _ptr(out["pos0"]),
+                                          _ptr(out["vel"]), _ptr(out["frc"])),
+              "psim_ens_get_state")
+        return {k: v.reshape(self.n, 3) for k, v in out.items()}
+
+    def replica_slice(self, r: int) -> slice:
+        return slice(int(self.offsets[r]), int(self.offsets[r + 1]))
+
+    def set_graph(self, steps_per_graph: int):
+        check(self.lib.psim_ens_set_graph(self._ctx, int(steps_per_graph)), "psim_ens_set_graph")
+
+    def step(self, nsteps: int = 1):
+        check(self.lib.psim_ens_step(self._ctx, int(nsteps)), "psim_ens_step")
+
+    def sync(self):
+        check(self.lib.psim_ens_sync(self._ctx), "psim_ens_sync")
+
+    def last_error(self):
+        code, i, j = C.c_int(), C.c_int(), C.c_int()
+        check(self.lib.psim_ens_last_error(self._ctx, C.byref(code), C.byref(i), C.byref(j)),
+              "psim_ens_last_error")
+        return code.value, i.value, j.value
+
+    def autotune(self, trial_steps: int = 5):
+        b, ms = C.c_int(), C.c_float()
+        check(self.lib.psim_ens_autotune(self._ctx, int(trial_steps), C.byref(b), C.byref(ms)),
+              "psim_ens_autotune")
+        return b.value, ms.value
