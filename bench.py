@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=25)
     ap.add_argument("--path", default="auto", choices=["auto", "sparse", "dense"])
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 512-replica batched-ensemble measurement")
