@@ -36,6 +36,18 @@
  *   psim_gpu_last_error    error::throw* (error.cpp:40-62), as a code
  *   psim_init_rsa          system::initParticles + initCheck
  *                          (systemInit.cpp:31-160), O(N) with a cell grid
+ *   psim_gpu_set_force_params  the plugin constructors' parameter reads
+ *                          (LJPotential.cpp:34-55, AOPotential.cpp:40-55),
+ *                          applied to an existing context
+ *   psim_gpu_track_start   analysisManager::initDisplacementTracker
+ *                          (analysisTracker.cpp:26-36)
+ *   psim_gpu_observables   getTemperature / meanDisplacement
+ *                          (analysisGeneral.cpp:26-56), trackedDisplacement
+ *                          (analysisTracker.cpp:67-85), coordination from
+ *                          updateInteractions (systemHandling.cpp:94-161,
+ *                          analysisOutput.cpp:116-126)
+ *   psim_gpu_clusters      analysisManager::findClusters / writeClusters
+ *                          (analysisCluster.cpp:26-100)
  */
 #ifndef PSIM_GPU_H
 #define PSIM_GPU_H
@@ -201,6 +213,90 @@ int psim_ens_last_error(psim_ens *ens, int *code, int *i, int *j);
 /* measured choice of the pair kernel's workgroup size (64/128/256); the
  * trial steps advance the ensemble */
 int psim_ens_autotune(psim_ens *ens, int trial_steps, int *chosen_block, float *ms_per_step);
+
+/* ---- analysis of the current state --------------------------------------- */
+
+/* Replace the force-module parameters (force_mask, lj_num, cutoff,
+ * lj_well_depth, yukawa_strength, debye_length, ao_well_depth) of a context;
+ * geometry, integrator and state are unchanged.  Other fields of params are
+ * ignored. */
+int psim_gpu_set_force_params(psim_ctx *ctx, const psim_params *params);
+
+/* Start the displacement tracker at the current positions: zero the
+ * periodic-crossing counters, which every later step updates. */
+int psim_gpu_track_start(psim_ctx *ctx);
+
+typedef struct psim_observables {
+    double temperature;       /* sum v^2 / 3 / N                             */
+    double mean_disp2;        /* sum |x - x0|^2 / N                          */
+    double msd;               /* sum |x + L*crossings - x_start|^2 / N (FP64);
+                                 NaN before psim_gpu_track_start            */
+    double mean_coordination; /* contacts / N                                */
+    long long contacts;       /* ordered pairs with r < contact_r, over the
+                                 27 cells of the force grid                 */
+    double time;              /* current time                                */
+    unsigned long long step;  /* step index                                  */
+} psim_observables;
+
+/* contact_r 0 skips the contact pass (coordination 0); the reference uses
+ * 1.2.  coord_by_orig (may be NULL) receives each particle's contacts. */
+int psim_gpu_observables(psim_ctx *ctx, double contact_r, psim_observables *out,
+                         int *coord_by_orig);
+
+/* Connected components of the contact graph (r < contact_r).  labels_by_orig
+ * (may be NULL) receives the smallest original index of each particle's
+ * component; n_clusters and mean_size count components of at least
+ * min_size members (the reference counts more than 4), mean by integer
+ * division. */
+int psim_gpu_clusters(psim_ctx *ctx, double contact_r, int min_size, int *labels_by_orig,
+                      int *n_clusters, int *mean_size);
+
+/* Pair-distance histogram over ordered pairs i != j (minimum image), bins
+ * [r_lo + b*width, r_lo + (b+1)*width), b < nbins (<= 2048).  sigma == 0:
+ * counts; sigma > 0: each pair adds the mass of a normal distribution
+ * N(r_ij, sigma^2) that falls in each bin (support cut at 8 sigma).  The
+ * reach r_lo + nbins*width + 8*sigma must not exceed half the box.  No
+ * reference counterpart (the structure observable of the fitting loss). */
+int psim_gpu_pair_histogram(psim_ctx *ctx, double r_lo, double width, int nbins,
+                            double sigma, double *hist);
+/* grad (AoS3, original order) = d(sum_b w[b] * hist[b]) / d positions,
+ * sigma > 0. */
+int psim_gpu_pair_histogram_grad(psim_ctx *ctx, double r_lo, double width, int nbins,
+                                 double sigma, const double *w, double *grad);
+
+/* ---- trajectory adjoint -------------------------------------------------------
+ * Reverse-mode derivative of a trajectory of nsteps G&B steps (in-kernel
+ * Philox noise, so the noise is fixed by the seed and step index) with
+ * respect to the pair-force parameters.  No reference counterpart: the
+ * reference has no derivatives; the forward steps are psim_gpu_step's.
+ * The derivative is pathwise: the set of pairs inside the cutoff is the one
+ * the forward trajectory used.  dtheta[PSIM_DP_*] receives dL/dparameter. */
+enum {
+    PSIM_DP_AO_WELL = 0,      /* AO wellDepth (polymer fraction phi_p)       */
+    PSIM_DP_AO_CUTOFF = 1,    /* cutOff through the AO coefficients          */
+    PSIM_DP_LJ_WELL = 2,      /* LJ wellDepth (epsilon)                      */
+    PSIM_DP_YUKAWA = 3,       /* yukawaStrength                              */
+    PSIM_DP_DEBYE = 4,        /* debyeLength                                 */
+    PSIM_DP_COUNT = 5
+};
+typedef struct psim_adjoint psim_adjoint;
+
+/* Takes the context's current state as the trajectory's start; keeps a
+ * checkpoint every checkpoint_every steps (device memory about
+ * (nsteps/checkpoint_every + 1) * 16 N doubles + checkpoint_every * 4 N). */
+int psim_adj_create(psim_ctx *ctx, int nsteps, int checkpoint_every, psim_adjoint **out);
+void psim_adj_destroy(psim_adjoint *adj);
+/* Re-take the start state from the context's current state. */
+int psim_adj_set_initial(psim_adjoint *adj);
+/* Run the trajectory from the start state with the context's current
+ * parameters; the context is left at step nsteps.  Returns 0 or a physics
+ * error code. */
+int psim_adj_forward(psim_adjoint *adj);
+/* gx = dL/dx at the last step (AoS3, original order).  Writes
+ * dtheta[PSIM_DP_COUNT] and, if gx0 is not NULL, dL/dx at the start.  The
+ * force parameters must be those of the last psim_adj_forward; the context
+ * is left at step nsteps. */
+int psim_adj_backward(psim_adjoint *adj, const double *gx, double *dtheta, double *gx0);
 
 /* Seeded random sequential addition of n spheres with pair distance >= dmin
  * in a periodic cube [0, box)^3 (host, O(n)).  Returns 0 or PSIM_ERR_INIT. */

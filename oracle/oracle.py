@@ -103,6 +103,21 @@ class Oracle:
         L.po_temperature.restype = C.c_double
         L.po_pair_histogram.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
                                         C.c_double, C.c_int, _dp]
+        L.po_track_update.argtypes = [C.c_int, C.c_double, _dp, _ip]
+        L.po_contacts.argtypes = [C.c_int, _dp, _ip, _ip, _ip, C.c_int, C.c_double,
+                                  C.c_double, C.c_double, _ip, C.c_void_p]
+        L.po_pair_histogram_smooth.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
+                                               C.c_double, C.c_int, C.c_double, _dp]
+        L.po_pair_histogram_grad.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
+                                             C.c_double, C.c_int, C.c_double, _dp, _dp]
+        L.po_pair_fnet_d.argtypes = [C.POINTER(ForceParams), C.c_double, C.c_double,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double), _dp]
+        L.po_adjoint.argtypes = [C.POINTER(ForceParams), C.POINTER(GBCoeffs), C.c_int,
+                                 C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                 C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp,
+                                 C.c_int, C.c_ulonglong, C.c_ulonglong, _dp, _dp,
+                                 C.c_void_p, C.c_void_p, C.c_int]
+        L.po_adjoint.restype = C.c_int
 
     # -- geometry / cells -------------------------------------------------
     def geometry(self, n, conc, scale, radius=0.5):
@@ -242,6 +257,70 @@ class Oracle:
         out = np.empty(nbins)
         self.lib.po_pair_histogram(len(pos), pos.ravel(), box, r_lo, width, nbins, out)
         return out
+
+    def track_update(self, box, pos0, pbc):
+        """analysisTracker.cpp:38-64 on AoS3 arrays; updates pbc in place."""
+        pos0 = _f64(pos0).reshape(-1)
+        assert pbc.dtype == np.int32 and pbc.flags.c_contiguous
+        self.lib.po_track_update(pos0.size // 3, float(box), pos0, pbc.reshape(-1))
+
+    def contacts(self, pos, S, cell_size, box, rc=1.2, radius=0.5, labels=False):
+        """Coordination by original index (and component labels) from the
+        reference's 27-cell contact scan on a fresh cell table."""
+        pos = _f64(pos).reshape(-1, 3)
+        n = len(pos)
+        key = self.hash(pos, cell_size, S)
+        perm, cs, ce = self.build_cells(key, S ** 3)
+        sorted4 = self.reorder(perm, pos, radius)
+        coord = np.empty(n, np.int32)
+        lab = np.empty(n, np.int32) if labels else None
+        self.lib.po_contacts(n, sorted4.ravel(), perm, cs, ce, int(S), float(cell_size),
+                             float(box), float(rc), coord,
+                             None if lab is None else lab.ctypes.data_as(C.c_void_p))
+        return (coord, lab) if labels else coord
+
+    def pair_histogram_smooth(self, pos, box, r_lo, width, nbins, sigma):
+        pos = _f64(pos).reshape(-1, 3)
+        out = np.empty(nbins)
+        self.lib.po_pair_histogram_smooth(len(pos), pos.ravel(), float(box), float(r_lo),
+                                          float(width), int(nbins), float(sigma), out)
+        return out
+
+    def pair_histogram_grad(self, pos, box, r_lo, width, nbins, sigma, w):
+        pos = _f64(pos).reshape(-1, 3)
+        out = np.empty(pos.size)
+        self.lib.po_pair_histogram_grad(len(pos), pos.ravel(), float(box), float(r_lo),
+                                        float(width), int(nbins), float(sigma), _f64(w),
+                                        out)
+        return out.reshape(-1, 3)
+
+    def pair_fnet_d(self, fp, r, size=1.0):
+        f, d = C.c_double(), C.c_double()
+        th = np.empty(5)
+        self.lib.po_pair_fnet_d(C.byref(fp), float(r), float(size), C.byref(f), C.byref(d),
+                                th)
+        return f.value, d.value, th
+
+    def adjoint(self, fp, c, state, nsteps, box, S, cell_size, gx, radius=0.5,
+                vel_freq=1000, seed=0, half_uv=None, nthreads=0):
+        """dL/dtheta (5), dL/dx_0 and x_K for nsteps from ``state`` (not modified)."""
+        n = state["pos"].shape[0]
+        if half_uv is None:
+            half_uv = float(int(box) // 2) if float(box).is_integer() else box / 2.0
+        arr = {k: _f64(state.get(k, np.zeros((n, 3)))).reshape(-1)
+               for k in ("pos", "pos0", "vel", "frc", "memx")}
+        dth = np.empty(5)
+        gx0 = np.empty(3 * n)
+        xk = np.empty(3 * n)
+        st = self.lib.po_adjoint(C.byref(fp), C.byref(c), n, float(box), float(half_uv),
+                                 int(S), float(cell_size), float(radius), int(vel_freq),
+                                 int(state.get("vel_counter", 0)),
+                                 float(state.get("time", 0.0)), arr["pos"], arr["pos0"],
+                                 arr["vel"], arr["frc"], arr["memx"], int(nsteps), int(seed),
+                                 int(state.get("step_index", 0)), _f64(gx).reshape(-1), dth,
+                                 gx0.ctypes.data_as(C.c_void_p),
+                                 xk.ctypes.data_as(C.c_void_p), int(nthreads))
+        return st, dth, gx0.reshape(-1, 3), xk.reshape(-1, 3)
 
 
 _ORACLE = None

@@ -737,3 +737,307 @@ void po_pair_histogram(int n, const double *pos, double box, double r_lo,
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Analysis (continued)                                                  */
+/* ------------------------------------------------------------------ */
+
+/* analysisManager::updateParticlePBC over all particles
+ * (analysisTracker.cpp:38-64); pos0 and pbc are AoS3 by original index. */
+void po_track_update(int n, double box, const double *pos0, int *pbc)
+{
+    for (int k = 0; k < 3 * n; k++) {
+        if (pos0[k] < 0) pbc[k] += 1;
+        else if (pos0[k] > box) pbc[k] -= 1;
+    }
+}
+
+/* Contacts r = sqrt(pbcDist) < rc over the 27 cells around each sorted
+ * particle (system::updateInteractions / iterateParticleInteractions,
+ * systemHandling.cpp:94-161).  coord is by original index; if labels is not
+ * NULL it receives the connected components of the contact graph
+ * (analysisManager::findClusters, analysisCluster.cpp:26-83) as the smallest
+ * original index of each component. */
+static int po_find(int *p, int x)
+{
+    while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
+    return x;
+}
+
+void po_contacts(int n, const double *sorted4, const int *perm, const int *cstart,
+                 const int *cend, int S, double cell_size, double box, double rc,
+                 int *coord, int *labels)
+{
+    int *parent = NULL;
+    if (labels) {
+        parent = (int *)malloc(sizeof(int) * (size_t)n);
+        for (int i = 0; i < n; i++) parent[i] = i;
+    }
+    int S2 = S * S;
+    for (int index = 0; index < n; index++) {
+        const double *a = sorted4 + 4 * index;
+        int cx = (int)floor(a[0] / cell_size);
+        int cy = (int)floor(a[1] / cell_size);
+        int cz = (int)floor(a[2] / cell_size);
+        int cnt = 0;
+        for (int x = -1; x <= 1; x++)
+            for (int y = -1; y <= 1; y++)
+                for (int z = -1; z <= 1; z++) {
+                    int rx = (cx + x) % S, ry = (cy + y) % S, rz = (cz + z) % S;
+                    rx = (rx < 0) ? rx + S : rx;
+                    ry = (ry < 0) ? ry + S : ry;
+                    rz = (rz < 0) ? rz + S : rz;
+                    int hash = rx + (S * ry) + (S2 * rz);
+                    int start = cstart[hash];
+                    if (start == -1) continue;
+                    int end = cend[hash];
+                    for (int i = start; i < end; i++) {
+                        if (i == index) continue;
+                        const double *b = sorted4 + 4 * i;
+                        double r2 = po_pbc_dist(a[0], a[1], a[2], b[0], b[1], b[2], box);
+                        double r = sqrt(r2);
+                        if (r < rc) {
+                            cnt++;
+                            if (parent) {
+                                int ra = po_find(parent, perm[index]);
+                                int rb = po_find(parent, perm[i]);
+                                if (ra < rb) parent[rb] = ra;
+                                else if (rb < ra) parent[ra] = rb;
+                            }
+                        }
+                    }
+                }
+        coord[perm[index]] = cnt;
+    }
+    if (labels) {
+        for (int i = 0; i < n; i++) labels[i] = po_find(parent, i);
+        free(parent);
+    }
+}
+
+/* Gaussian-smoothed pair-distance histogram over ordered pairs i != j
+ * (minimum image): bin b = [r_lo + b w, r_lo + (b+1) w) receives
+ * Phi((e_{b+1} - r)/sigma) - Phi((e_b - r)/sigma); sigma == 0 counts.
+ * Brute force O(n^2); pos is AoS3. */
+static double po_ncdf(double t) { return 0.5 * erfc(-t / sqrt(2.0)); }
+static double po_npdf(double t) { return exp(-0.5 * t * t) / sqrt(2.0 * M_PI); }
+
+void po_pair_histogram_smooth(int n, const double *pos, double box, double r_lo,
+                              double width, int nbins, double sigma, double *hist)
+{
+    for (int b = 0; b < nbins; b++) hist[b] = 0.0;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            if (i == j) continue;
+            double r = sqrt(po_pbc_dist(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2],
+                                        pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], box));
+            if (sigma == 0.0) {
+                double q = floor((r - r_lo) / width);
+                if (q >= 0 && q < nbins) hist[(int)q] += 1.0;
+                continue;
+            }
+            for (int b = 0; b < nbins; b++) {
+                double lo = r_lo + width * b, hi = r_lo + width * (b + 1);
+                hist[b] += po_ncdf((hi - r) / sigma) - po_ncdf((lo - r) / sigma);
+            }
+        }
+}
+
+/* grad (AoS3) = d(sum_b w_b hist_b)/d pos for sigma > 0 (brute force). */
+void po_pair_histogram_grad(int n, const double *pos, double box, double r_lo,
+                            double width, int nbins, double sigma, const double *w,
+                            double *grad)
+{
+    double half = box / 2.0;
+    for (int k = 0; k < 3 * n; k++) grad[k] = 0.0;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            if (i == j) continue;
+            double d[3];
+            for (int k = 0; k < 3; k++) {
+                double e = pos[3 * j + k] - pos[3 * i + k];
+                if (e > half) e -= box;
+                else if (e < -half) e += box;
+                d[k] = e;
+            }
+            double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            double s = 0.0;   /* sum_b w_b dm_b/dr */
+            for (int b = 0; b < nbins; b++) {
+                double lo = r_lo + width * b, hi = r_lo + width * (b + 1);
+                s += w[b] * (po_npdf((lo - r) / sigma) - po_npdf((hi - r) / sigma)) / sigma;
+            }
+            /* the ordered pairs (i,j) and (j,i) both move with x_i */
+            for (int k = 0; k < 3; k++) grad[3 * i + k] += -2.0 * s * d[k] / r;
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* Derivatives of the pair forces and the trajectory adjoint              */
+/* ------------------------------------------------------------------ */
+
+/* fNet(r), dfNet/dr and dfNet/dtheta summed over the enabled modules, with
+ * theta = (AO wellDepth, cutOff through the AO coefficients, LJ wellDepth,
+ * yukawaStrength, debyeLength); differentiated from po_f_lj / po_f_ao /
+ * po_f_cal (LJPotential.cpp:97-109, AOPotential.cpp:50-55, 95-102,
+ * Calibration.cpp:75-81). */
+void po_pair_fnet_d(const po_force_params *fp, double r, double size, double *f,
+                    double *dfdr, double dth[5])
+{
+    double F = 0.0, D = 0.0;
+    for (int k = 0; k < 5; k++) dth[k] = 0.0;
+    if (fp->mask & PO_CAL) {
+        F += -36.0 * pow(r, -37);
+        D += 36.0 * 37.0 * pow(r, -38);
+    }
+    if (fp->mask & PO_LJ) {
+        double n = fp->lj_num, lam = fp->debye_length, A = fp->yukawa_strength;
+        double kte = fp->kT * fp->lj_well_depth;
+        double lj = pow(size / r, n);
+        double attract = 4.0 * n * (2.0 * lj * lj - lj) / r;
+        double dattract = 4.0 * n * ((-2.0 * n * 2.0 * lj * lj + n * lj) / (r * r)
+                                     - (2.0 * lj * lj - lj) / (r * r));
+        double e = exp(-r / lam);
+        double yb = e * (lam + r) / (r * r);
+        double dyb = (-e / lam * (lam + r) + e) / (r * r) - 2.0 * e * (lam + r) / (r * r * r);
+        F += -kte * (attract + A * yb);
+        D += -kte * (dattract + A * dyb);
+        dth[2] += -fp->kT * (attract + A * yb);
+        dth[3] += -kte * yb;
+        dth[4] += -kte * A * (e * r / (lam * lam) * (lam + r) + e) / (r * r);
+    }
+    if (fp->mask & PO_AO) {
+        double c = fp->cutoff, phi = fp->ao_well_depth, kT = fp->kT;
+        double Kc = -kT * pow(c / (c - 1.0), 3);
+        double a1 = Kc * phi, a2 = -3.0 / (2.0 * c), a3 = 1.0 / (2.0 * c * c * c);
+        double C1 = -a1 * a2, C2 = -3.0 * a1 * a3;
+        F += -(36.0 * pow(r, -38) + C1 / r + C2 * r);
+        D += 36.0 * 38.0 * pow(r, -39) + C1 / (r * r) - C2;
+        /* d/dphi */
+        dth[0] += -((-Kc * a2) / r + (-3.0 * Kc * a3) * r);
+        /* d/dc: a1' = a1 (3/c - 3/(c-1)), a2' = 3/(2c^2), a3' = -3/(2c^4) */
+        double da1 = a1 * (3.0 / c - 3.0 / (c - 1.0));
+        double da2 = 3.0 / (2.0 * c * c), da3 = -3.0 / (2.0 * c * c * c * c);
+        double dC1 = -(da1 * a2 + a1 * da2), dC2 = -3.0 * (da1 * a3 + a1 * da3);
+        dth[1] += -(dC1 / r + dC2 * r);
+    }
+    *f = F;
+    *dfdr = D;
+}
+
+/* Reverse-mode derivative of nsteps of po_run (Philox noise keyed by
+ * seed / step / original index) for a loss with gradient gx = dL/dx_K:
+ * dtheta[5] (see po_pair_fnet_d) and, if gx0 is not NULL, dL/dx_0.  Stores
+ * every step's positions (test sizes).  pos_out (may be NULL) receives x_K.
+ * Recurrence: lambda_n = alpha_n a_{n+1} - c2 a_{n+2},
+ * a_n = beta_n a_{n+1} - coeff0 a_{n+2} + J(x_n) lambda_n,
+ * dL/dtheta = sum_n lambda_n . dF(x_n)/dtheta. */
+int po_adjoint(const po_force_params *fp, const po_gb_coeffs *c, int n, double box,
+               double half_uv, int S, double cell_size, double radius, int vel_freq,
+               int vel_counter, double current_time, const double *pos_in,
+               const double *pos0_in, const double *vel_in, const double *frc_in,
+               const double *memx_in, int nsteps, unsigned long long seed,
+               unsigned long long step_index, const double *gx, double *dtheta, double *gx0,
+               double *pos_out, int nthreads)
+{
+    size_t n3 = 3 * (size_t)n;
+    int ncell = S * S * S;
+    double *traj = (double *)malloc(sizeof(double) * n3 * (size_t)nsteps);
+    int *first = (int *)malloc(sizeof(int) * (size_t)nsteps);
+    double *pos = (double *)malloc(sizeof(double) * n3);
+    double *pos0 = (double *)malloc(sizeof(double) * n3);
+    double *vel = (double *)malloc(sizeof(double) * n3);
+    double *frc = (double *)malloc(sizeof(double) * n3);
+    double *frc0 = (double *)calloc(n3, sizeof(double));
+    double *memx = (double *)malloc(sizeof(double) * n3);
+    memcpy(pos, pos_in, sizeof(double) * n3);
+    memcpy(pos0, pos0_in, sizeof(double) * n3);
+    memcpy(vel, vel_in, sizeof(double) * n3);
+    memcpy(frc, frc_in, sizeof(double) * n3);
+    memcpy(memx, memx_in, sizeof(double) * n3);
+    int status = 0, eij[2];
+    for (int s = 0; s < nsteps && !status; s++) {
+        memcpy(traj + (size_t)s * n3, pos, sizeof(double) * n3);
+        first[s] = (current_time == 0);
+        status = po_run(fp, c, n, box, half_uv, S, cell_size, radius, vel_freq, &vel_counter,
+                        &current_time, pos, pos0, vel, frc, frc0, memx, 1, NULL, seed,
+                        &step_index, eij, nthreads);
+    }
+    if (pos_out) memcpy(pos_out, pos, sizeof(double) * n3);
+    double m = 1.0 / c->mass, dt2 = c->dt * c->dt;
+    double c1 = m * dt2 * c->coeff1, c2 = m * dt2 * c->coeff2;
+    double alpha_first = c->coeff3 * dt2 * m, alpha_normal = c1 + c2;
+    double *a0 = (double *)malloc(sizeof(double) * n3);
+    double *a1 = (double *)malloc(sizeof(double) * n3);
+    double *a2 = (double *)calloc(n3, sizeof(double));
+    double *lam = (double *)malloc(sizeof(double) * n3);
+    double *lamp = (double *)malloc(sizeof(double) * n3);
+    double *jl = (double *)malloc(sizeof(double) * n3);
+    int *key = (int *)malloc(sizeof(int) * (size_t)n);
+    int *perm = (int *)malloc(sizeof(int) * (size_t)n);
+    int *cs = (int *)malloc(sizeof(int) * (size_t)ncell);
+    int *ce = (int *)malloc(sizeof(int) * (size_t)ncell);
+    double *sorted = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+    double th[5] = {0, 0, 0, 0, 0};
+    memcpy(a1, gx, sizeof(double) * n3);
+    double aK = first[nsteps - 1] ? alpha_first : alpha_normal;
+    for (size_t k = 0; k < n3; k++) lam[k] = aK * a1[k];
+    for (int s = nsteps - 1; s >= 0 && !status; s--) {
+        const double *x = traj + (size_t)s * n3;
+        po_hash(n, x, cell_size, S, key);
+        po_build_cells(n, key, ncell, perm, cs, ce);
+        po_reorder(n, perm, x, radius, sorted);
+        for (size_t k = 0; k < n3; k++) jl[k] = 0.0;
+        for (int index = 0; index < n; index++) {
+            const double *a = sorted + 4 * index;
+            int oi = perm[index];
+            int cx = po_cell_1d(a[0], cell_size, S);
+            int cy = po_cell_1d(a[1], cell_size, S);
+            int cz = po_cell_1d(a[2], cell_size, S);
+            for (int ox = -1; ox <= 1; ox++)
+                for (int oy = -1; oy <= 1; oy++)
+                    for (int oz = -1; oz <= 1; oz++) {
+                        int rx = (cx + ox + S) % S, ry = (cy + oy + S) % S, rz = (cz + oz + S) % S;
+                        int h = rx + S * ry + S * S * rz;
+                        if (cs[h] == -1) continue;
+                        for (int i = cs[h]; i < ce[h]; i++) {
+                            if (i == index) continue;
+                            const double *b = sorted + 4 * i;
+                            double r2 = po_pbc_dist(a[0], a[1], a[2], b[0], b[1], b[2], box);
+                            if (!(r2 < fp->cutoff * fp->cutoff)) continue;
+                            double r = sqrt(r2);
+                            double u[3];
+                            po_unit_vector(a[0], a[1], a[2], b[0], b[1], b[2], u, r, box,
+                                           half_uv);
+                            double d[3] = {u[0] * r, u[1] * r, u[2] * r};
+                            double f, dfdr, dth[5];
+                            po_pair_fnet_d(fp, r, a[3] + b[3], &f, &dfdr, dth);
+                            double g = f / r, gp = (dfdr - g) / r;
+                            int oj = perm[i];
+                            double e[3], dd = 0.0, ld = 0.0;
+                            for (int k = 0; k < 3; k++) {
+                                e[k] = lam[3 * oj + k] - lam[3 * oi + k];
+                                dd += d[k] * e[k];
+                                ld += lam[3 * oi + k] * d[k];
+                            }
+                            for (int k = 0; k < 3; k++)
+                                jl[3 * oi + k] += g * e[k] + gp / r * dd * d[k];
+                            for (int q = 0; q < 5; q++) th[q] += dth[q] / r * ld;
+                        }
+                    }
+        }
+        double beta = first[s] ? 1.0 : 1.0 + c->coeff0;
+        for (size_t k = 0; k < n3; k++) a0[k] = beta * a1[k] - c->coeff0 * a2[k] + jl[k];
+        if (s >= 1) {
+            double ap = first[s - 1] ? alpha_first : alpha_normal;
+            for (size_t k = 0; k < n3; k++) lamp[k] = ap * a0[k] - c2 * a1[k];
+        }
+        double *t = a2; a2 = a1; a1 = a0; a0 = t;
+        t = lam; lam = lamp; lamp = t;
+    }
+    for (int q = 0; q < 5; q++) dtheta[q] = th[q];
+    if (gx0) memcpy(gx0, a1, sizeof(double) * n3);
+    free(traj); free(first); free(pos); free(pos0); free(vel); free(frc); free(frc0);
+    free(memx); free(a0); free(a1); free(a2); free(lam); free(lamp); free(jl); free(key);
+    free(perm); free(cs); free(ce); free(sorted);
+    return status;
+}

@@ -25,6 +25,15 @@
  *   po_run                 composition of the above, vs reference
  *                          trajectories (tolerance: sort order)         pinned
  *   analysis (T, MSD, g(r))closed forms (ideal-gas g(r)=1, OU MSD)      pinned
+ *   po_contacts            brute-force neighbour count, union-find of
+ *                          brute-force contacts                          pinned
+ *   po_pair_histogram_smooth  sigma -> 0 equals the plain histogram;
+ *                          total mass = pairs in range                   pinned
+ *   po_pair_histogram_grad central differences of the histogram          pinned
+ *   po_pair_fnet_d         central differences of po_forces on two
+ *                          particles (r and each parameter)             pinned
+ *   po_adjoint             central differences of the loss of po_run
+ *                          trajectories (pair set unchanged)            pinned
  */
 #ifndef PSIM_ORACLE_H
 #define PSIM_ORACLE_H
@@ -151,6 +160,38 @@ double po_temperature(int n, const double *vel);
  * [r_lo, r_lo + nbins*width): counts[b] (each unordered pair once). */
 void po_pair_histogram(int n, const double *pos, double box, double r_lo,
                        double width, int nbins, double *counts);
+
+/* analysisTracker.cpp:38-64: periodic-crossing counters (AoS3 ints by
+ * original index) from the re-imaged previous positions. */
+void po_track_update(int n, double box, const double *pos0, int *pbc);
+/* systemHandling.cpp:94-161 contact scan (r < rc over 27 cells) on the
+ * reference's sorted arrays: coord by original index; labels (may be NULL)
+ * = smallest original index of each contact component
+ * (analysisCluster.cpp:26-83). */
+void po_contacts(int n, const double *sorted4, const int *perm, const int *cstart,
+                 const int *cend, int S, double cell_size, double box, double rc,
+                 int *coord, int *labels);
+/* Gaussian-smoothed pair histogram over ordered pairs (brute force) and
+ * the gradient of sum_b w_b hist_b with respect to the positions. */
+void po_pair_histogram_smooth(int n, const double *pos, double box, double r_lo,
+                              double width, int nbins, double sigma, double *hist);
+void po_pair_histogram_grad(int n, const double *pos, double box, double r_lo,
+                            double width, int nbins, double sigma, const double *w,
+                            double *grad);
+/* fNet, dfNet/dr and dfNet/dtheta (theta = AO wellDepth, cutOff via the AO
+ * coefficients, LJ wellDepth, yukawaStrength, debyeLength). */
+void po_pair_fnet_d(const po_force_params *fp, double r, double size, double *f,
+                    double *dfdr, double dth[5]);
+/* Trajectory adjoint: nsteps of po_run with Philox noise, loss gradient gx
+ * at the last step -> dtheta[5], dL/dx_0 (gx0, may be NULL), x_K (pos_out,
+ * may be NULL). */
+int po_adjoint(const po_force_params *fp, const po_gb_coeffs *c, int n, double box,
+               double half_uv, int S, double cell_size, double radius, int vel_freq,
+               int vel_counter, double current_time, const double *pos_in,
+               const double *pos0_in, const double *vel_in, const double *frc_in,
+               const double *memx_in, int nsteps, unsigned long long seed,
+               unsigned long long step_index, const double *gx, double *dtheta, double *gx0,
+               double *pos_out, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -8,12 +8,12 @@ importing the engine loads it and fails loudly when it is missing.
 """
 from ._lib import (ERR_BOUNDS, ERR_INIT, ERR_NAN, ERR_OVERLAP, FORCE_AO, FORCE_CAL,  # noqa: F401
                    FORCE_LJ, PsimError, load)
-from .engine import (AOPotential, BrownianIntegrator, Calibration, Config, Ensemble,  # noqa: F401
+from .engine import (Adjoint, AOPotential, BrownianIntegrator, Calibration, Config, Ensemble,  # noqa: F401
                      ForceSet, GpuSystem, LennardJones, PLUGINS, Replica, SystemGeometry,
                      rsa_positions)
 
 __all__ = [
-    "AOPotential", "BrownianIntegrator", "Calibration", "Config", "Ensemble", "ForceSet",
+    "Adjoint", "AOPotential", "BrownianIntegrator", "Calibration", "Config", "Ensemble", "ForceSet",
     "GpuSystem", "Replica",
     "LennardJones", "PLUGINS", "SystemGeometry", "rsa_positions", "PsimError", "load",
 ]

@@ -39,6 +39,15 @@ class PsimReplica(C.Structure):
                 ("lj_num", C.c_int), ("reserved", C.c_int)]
 
 
+class Observables(C.Structure):
+    _fields_ = [("temperature", C.c_double), ("mean_disp2", C.c_double),
+                ("msd", C.c_double), ("mean_coordination", C.c_double),
+                ("contacts", C.c_longlong), ("time", C.c_double), ("step", C.c_ulonglong)]
+
+
+DP_AO_WELL, DP_AO_CUTOFF, DP_LJ_WELL, DP_YUKAWA, DP_DEBYE, DP_COUNT = range(6)
+
+
 class KernelTimes(C.Structure):
     _fields_ = [("force_ms", C.c_double), ("force_launches", C.c_int),
                 ("integrate_ms", C.c_double), ("integrate_launches", C.c_int),
@@ -57,6 +66,10 @@ EXPORTS = (
     "psim_ens_create", "psim_ens_destroy", "psim_ens_total_particles", "psim_ens_set_state",
     "psim_ens_get_state", "psim_ens_set_graph", "psim_ens_step", "psim_ens_sync",
     "psim_ens_last_error", "psim_ens_autotune",
+    "psim_gpu_set_force_params", "psim_gpu_track_start", "psim_gpu_observables",
+    "psim_gpu_clusters", "psim_gpu_pair_histogram", "psim_gpu_pair_histogram_grad",
+    "psim_adj_create", "psim_adj_destroy", "psim_adj_set_initial", "psim_adj_forward",
+    "psim_adj_backward",
 )
 
 _vp = C.c_void_p
@@ -95,6 +108,19 @@ _SIGS = {
     "psim_ens_sync": ([_vp], C.c_int),
     "psim_ens_last_error": ([_vp, _ip, _ip, _ip], C.c_int),
     "psim_ens_autotune": ([_vp, C.c_int, _ip, C.POINTER(C.c_float)], C.c_int),
+    "psim_gpu_set_force_params": ([_vp, C.POINTER(PsimParams)], C.c_int),
+    "psim_gpu_track_start": ([_vp], C.c_int),
+    "psim_gpu_observables": ([_vp, C.c_double, C.POINTER(Observables), _ip], C.c_int),
+    "psim_gpu_clusters": ([_vp, C.c_double, C.c_int, _ip, _ip, _ip], C.c_int),
+    "psim_gpu_pair_histogram": ([_vp, C.c_double, C.c_double, C.c_int, C.c_double, _dp],
+                                C.c_int),
+    "psim_gpu_pair_histogram_grad": ([_vp, C.c_double, C.c_double, C.c_int, C.c_double,
+                                      _dp, _dp], C.c_int),
+    "psim_adj_create": ([_vp, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
+    "psim_adj_destroy": ([_vp], None),
+    "psim_adj_set_initial": ([_vp], C.c_int),
+    "psim_adj_forward": ([_vp], C.c_int),
+    "psim_adj_backward": ([_vp, _dp, _dp, _dp], C.c_int),
 }
 
 _LIB = None

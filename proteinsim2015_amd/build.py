@@ -12,6 +12,7 @@ Test infrastructure built alongside (never loaded by the product):
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -52,19 +53,23 @@ def _run(cmd, verbose):
 
 def build_product(verbose: bool = False) -> str:
     hipcc = _hipcc()
-    objs = []
     tmp = os.path.join(PKG, "_obj")
     os.makedirs(tmp, exist_ok=True)
-    for src in ("psim_engine.hip", "psim_ensemble.hip"):
+    jobs = []
+    for src in ("psim_engine.hip", "psim_ensemble.hip", "psim_analysis.hip",
+                "psim_adjoint.hip"):
         obj = os.path.join(tmp, src + ".o")
-        _run([hipcc, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c",
-              os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        jobs.append((obj, [hipcc, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c",
+                           os.path.join(CSRC, src), "-o", obj]))
     for src in ("psim_host.cpp",):
         obj = os.path.join(tmp, src + ".o")
-        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-I", INCLUDE, "-c",
-              os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        jobs.append((obj, ["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-I",
+                           INCLUDE, "-c", os.path.join(CSRC, src), "-o", obj]))
+    # the translation units are independent: compile them side by side
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for f in [ex.submit(_run, cmd, verbose) for _, cmd in jobs]:
+            f.result()
+    objs = [obj for obj, _ in jobs]
     out = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs, "-lgomp"],
          verbose)

@@ -1,0 +1,519 @@
+// psim_adjoint.hip -- reverse-mode derivative of a G&B trajectory with
+// respect to the pair-force parameters, for structure fitting.
+//
+// With fixed noise, the positions follow (brownianIntegrator.cpp:191-296)
+//   x_1     = x_0 + v_0 coEff1 dt + alpha_0 F(x_0) + noise            (first step)
+//   x_{n+1} = (1+c0) x_n - c0 x_{n-1} + (c1+c2) F(x_n) - c2 F(x_{n-1}) + noise
+// (c0 = coEff0, c1 = dt^2 coEff1 / m, c2 = dt^2 coEff2 / m,
+// alpha_0 = dt^2 coEff3 / m; the periodic wrap is a translation and the
+// velocity step leaves the positions as setPos does).  For a loss L(x_K),
+// with a_n = dL/dx_n and a_{K+1} = 0:
+//   lambda_n = alpha_n a_{n+1} - c2 a_{n+2}       (alpha_n = c1 + c2 after step 0)
+//   a_n      = beta_n a_{n+1} - c0 a_{n+2} + J_n lambda_n   (beta_0 = 1 on a
+//                                                           first step, else 1 + c0)
+//   dL/dtheta = sum_n lambda_n . dF(x_n)/dtheta
+// J_n = dF/dx at x_n (symmetric): for a pair force F_i = sum_j g(r) d_ij
+// with d_ij = x_j - x_i, (J lambda)_i = sum_j g (lambda_j - lambda_i)
+//   + (g'/r) (d . (lambda_j - lambda_i)) d.
+// The derivative is pathwise: the set of pairs inside the cutoff is the one
+// the forward trajectory used (a change of the cutoff moves no pair across
+// it), which is the exact derivative of the discrete trajectory wherever no
+// pair sits exactly at the cutoff.
+//
+// Memory: a checkpoint of the full state every `every` steps; the backward
+// pass recomputes each segment from its checkpoint (the in-kernel Philox
+// noise is keyed by step and particle, so the recomputation is bit-identical)
+// and records the positions and cell tables of that segment only.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "psim_ctx.hpp"
+#include "psim_pairs.hpp"
+
+using namespace psim;
+using eng::blocks_for;
+using eng::dalloc;
+
+namespace {
+
+constexpr int kAdjBlock = 256;
+constexpr int kNTheta = 5;   // AO wellDepth, AO cutOff, LJ wellDepth, yukawaStrength, debyeLength
+
+// Host-side derivatives of the pair constants (AOPotential.cpp:50-55).
+struct AdjPot {
+    double neg_kt;                      // -kT
+    double ao_dc1_dphi, ao_dc2_dphi;    // dC1/dwellDepth, dC2/dwellDepth
+    double ao_dc1_dc, ao_dc2_dc;        // dC1/dcutOff, dC2/dcutOff
+};
+
+AdjPot make_adj(const psim_params &P)
+{
+    AdjPot q{};
+    q.neg_kt = -P.kT;
+    const double rc = P.cutoff;
+    const double K = -P.kT * std::pow(rc / (rc - 1.0), 3);
+    const double a1 = K * P.ao_well_depth;
+    const double a2 = -3.0 / (2.0 * rc);
+    const double a3 = 1.0 / (2.0 * rc * rc * rc);
+    q.ao_dc1_dphi = -K * a2;
+    q.ao_dc2_dphi = -3.0 * K * a3;
+    const double da1 = -3.0 * a1 / (rc * (rc - 1.0));
+    const double da2 = 3.0 / (2.0 * rc * rc);
+    const double da3 = -3.0 / (2.0 * rc * rc * rc * rc);
+    q.ao_dc1_dc = -(da1 * a2 + a1 * da2);
+    q.ao_dc2_dc = -3.0 * (da1 * a3 + a1 * da3);
+    return q;
+}
+
+// fNet(r), dfNet/dr and dfNet/dtheta of the enabled modules (the forward
+// expressions of pair_fnet, LJPotential.cpp:97-109, AOPotential.cpp:95-102,
+// Calibration.cpp:75-81).
+template <int MASK>
+__device__ __forceinline__ void pair_fnet_d(const Pot &p, const AdjPot &q, double r,
+                                            double rinv, double size, double &f, double &fp,
+                                            double (&dth)[kNTheta])
+{
+    f = 0.0;
+    fp = 0.0;
+#pragma unroll
+    for (int k = 0; k < kNTheta; ++k) dth[k] = 0.0;
+    const double r2inv = rinv * rinv;
+    if (MASK & F_CAL) {
+        const double r37 = pow_const<37>(rinv);
+        f += -(36.0 * r37);
+        fp += 1332.0 * r37 * rinv;
+    }
+    if (MASK & F_LJ) {
+        const double n = (double)p.lj_num;
+        const double lj = lj_power(size * rinv, p.lj_num);
+        const double attract = fma(2.0, lj, -1.0) * (p.lj_four_n * rinv * lj);
+        const double dattract = -p.lj_four_n * lj * r2inv * (n * fma(4.0, lj, -1.0) + fma(2.0, lj, -1.0));
+        const double lam = p.debye_len;
+        const double e = exp_neg(r * p.debye_inv);
+        const double yb = e * (r2inv * (lam + r));   // repel / yukawaStrength
+        const double repel = yb * p.yuk_a;
+        const double drepel = p.yuk_a * e * r2inv * (-r * p.debye_inv - 2.0 * lam * rinv - 2.0);
+        f += p.lj_neg_kt_eps * (attract + repel);
+        fp += p.lj_neg_kt_eps * (dattract + drepel);
+        dth[2] += q.neg_kt * (attract + repel);
+        dth[3] += p.lj_neg_kt_eps * yb;
+        dth[4] += p.lj_neg_kt_eps * p.yuk_a * e * r2inv * (r * (lam + r) * p.debye_inv * p.debye_inv + 1.0);
+    }
+    if (MASK & F_AO) {
+        const double r38 = pow_const<19>(r2inv);
+        f += -(36.0 * r38 + p.ao_c1 * rinv + p.ao_c2 * r);
+        fp += 1368.0 * r38 * rinv + p.ao_c1 * r2inv - p.ao_c2;
+        dth[0] += -(q.ao_dc1_dphi * rinv + q.ao_dc2_dphi * r);
+        dth[1] += -(q.ao_dc1_dc * rinv + q.ao_dc2_dc * r);
+    }
+}
+
+// Pairs of slot s inside the cutoff, with the forward kernels' neighbour
+// test and displacement (psim_pairs.hpp gather_force / drain_hits).
+template <bool EXACT, class Fn>
+__device__ void visit_pairs(const Geo &g, const Pot &p, const double *pos, const int *cstart,
+                            int s, Fn fn)
+{
+    const int N = g.n, S = g.S;
+    const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
+    const int cx = cell_1d(xi, g.cell, S), cy = cell_1d(yi, g.cell, S),
+              cz = cell_1d(zi, g.cell, S);
+    for (int ox = -1; ox <= 1; ++ox) {
+        int rx = cx + ox, kx = 0;
+        if (rx < 0) { rx += S; kx = 1; } else if (rx >= S) { rx -= S; kx = 2; }
+        const double xs = shifted(xi, kx, g.L);
+        for (int oy = -1; oy <= 1; ++oy) {
+            int ry = cy + oy, ky = 0;
+            if (ry < 0) { ry += S; ky = 1; } else if (ry >= S) { ry -= S; ky = 2; }
+            const double ys = shifted(yi, ky, g.L);
+            for (int oz = -1; oz <= 1; ++oz) {
+                int rz = cz + oz, kz = 0;
+                if (rz < 0) { rz += S; kz = 1; } else if (rz >= S) { rz -= S; kz = 2; }
+                const double zs = shifted(zi, kz, g.L);
+                const int h = g.c0 + rx + S * ry + g.S2 * rz;
+                for (int j = cstart[h]; j < cstart[h + 1]; ++j) {
+                    if (j == s) continue;
+                    const double xj = pos[j], yj = pos[N + j], zj = pos[2 * N + j];
+                    double dx, dy, dz, r2, r, rinv;
+                    if (!EXACT) {
+                        dx = xj - xs;
+                        dy = yj - ys;
+                        dz = zj - zs;
+                        r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                        if (!(r2 < p.rc2_hi)) continue;
+                        if (r2 >= p.rc2_lo &&
+                            !(pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc) < p.rc2))
+                            continue;
+                        rinv = rsqrt_nr(r2);
+                        r = r2 * rinv;
+                    } else {
+                        r2 = pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc);
+                        if (!(r2 < p.rc2)) continue;
+                        r = sqrt(r2);
+                        rinv = 1.0 / r;
+                        dx = min_image_uv(xj - xi, g.L, g.half_uv);
+                        dy = min_image_uv(yj - yi, g.L, g.half_uv);
+                        dz = min_image_uv(zj - zi, g.L, g.half_uv);
+                    }
+                    fn(j, dx, dy, dz, r, rinv);
+                }
+            }
+        }
+    }
+}
+
+// One backward step at the recorded state x_n.
+template <int MASK, bool EXACT>
+__global__ __launch_bounds__(kAdjBlock) void k_adj_step(
+    Geo g, Pot p, AdjPot q, double radius, const double *pos, const int *id,
+    const int *cstart, const double *lam, const double *a1, const double *a2, double *a0,
+    double *lam_prev, double beta, double coeff0, double alpha_prev, double c2, double *tpart)
+{
+    const int s = blockIdx.x * kAdjBlock + threadIdx.x;
+    const int N = g.n;
+    double th[kNTheta] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (s < N) {
+        const int o = id[s];
+        const double lx = lam[o], ly = lam[N + o], lz = lam[2 * N + o];
+        const double size = 2.0 * radius;
+        double jx = 0.0, jy = 0.0, jz = 0.0;
+        visit_pairs<EXACT>(g, p, pos, cstart, s,
+                           [&](int j, double dx, double dy, double dz, double r, double rinv) {
+            const int oj = id[j];
+            const double ex = lam[oj] - lx, ey = lam[N + oj] - ly, ez = lam[2 * N + oj] - lz;
+            double f, fp, dth[kNTheta];
+            pair_fnet_d<MASK>(p, q, r, rinv, size, f, fp, dth);
+            const double gr = f * rinv;
+            const double gp = (fp - gr) * rinv;
+            const double dd = dx * ex + dy * ey + dz * ez;
+            const double k2 = gp * rinv * dd;
+            jx += gr * ex + k2 * dx;
+            jy += gr * ey + k2 * dy;
+            jz += gr * ez + k2 * dz;
+            const double ld = (lx * dx + ly * dy + lz * dz) * rinv;
+#pragma unroll
+            for (int k = 0; k < kNTheta; ++k) th[k] += dth[k] * ld;
+        });
+        const double n0x = beta * a1[o] - coeff0 * a2[o] + jx;
+        const double n0y = beta * a1[N + o] - coeff0 * a2[N + o] + jy;
+        const double n0z = beta * a1[2 * N + o] - coeff0 * a2[2 * N + o] + jz;
+        a0[o] = n0x;
+        a0[N + o] = n0y;
+        a0[2 * N + o] = n0z;
+        if (lam_prev) {
+            lam_prev[o] = alpha_prev * n0x - c2 * a1[o];
+            lam_prev[N + o] = alpha_prev * n0y - c2 * a1[N + o];
+            lam_prev[2 * N + o] = alpha_prev * n0z - c2 * a1[2 * N + o];
+        }
+    }
+    __shared__ double red[kNTheta][kAdjBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kNTheta; ++k) {
+        double x = th[k];
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) x += __shfl_down(x, d, 64);
+        if (lane == 0) red[k][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNTheta) {
+        double t = 0.0;
+        for (int k = 0; k < kAdjBlock / 64; ++k) t += red[threadIdx.x][k];
+        tpart[(size_t)blockIdx.x * kNTheta + threadIdx.x] = t;
+    }
+}
+
+// acc[k] += sum over blocks of tpart[b][k], in block order (deterministic)
+__global__ void k_theta_accum(const double *tpart, int nb, double *acc)
+{
+    const int k = threadIdx.x;
+    if (k >= kNTheta) return;
+    double t = 0.0;
+    for (int b = 0; b < nb; ++b) t += tpart[(size_t)b * kNTheta + k];
+    acc[k] += t;
+}
+
+// lam = alpha * a1 (the last step: a_{K+1} = 0)
+__global__ void k_scale(int n3, double alpha, const double *a, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) out[i] = alpha * a[i];
+}
+
+}  // namespace
+
+struct psim_adjoint {
+    psim_ctx *c = nullptr;
+    int K = 0, every = 0, ncp = 0;
+    struct Meta {
+        double time;
+        int vel_counter;
+        unsigned long long step;
+    };
+    std::vector<Meta> meta;        // ncp + 1 (the last = final state)
+    double *cp_d = nullptr;        // (ncp + 1) x 15N: pos, pos0, frc, mem, vel
+    int *cp_i = nullptr;           // (ncp + 1) x (N + ncell + 1): id, cstart
+    double *rec_pos = nullptr;     // every x 3N
+    int *rec_i = nullptr;          // every x (N + ncell + 1)
+    double *a[3] = {nullptr, nullptr, nullptr};
+    double *lam[2] = {nullptr, nullptr};
+    double *tpart = nullptr, *acc = nullptr;
+    std::vector<int> first;        // per step: first-step flag
+    bool have_forward = false;
+    psim_params prm_used{};
+};
+
+namespace {
+
+size_t cp_stride_d(const psim_ctx *c) { return 15 * (size_t)c->n; }
+size_t cp_stride_i(const psim_ctx *c) { return (size_t)c->n + c->ncell + 1; }
+
+int save_cp(psim_adjoint *A, int k)
+{
+    psim_ctx *c = A->c;
+    const size_t n3 = 3 * (size_t)c->n;
+    double *d = A->cp_d + (size_t)k * cp_stride_d(c);
+    int *ip = A->cp_i + (size_t)k * cp_stride_i(c);
+    const double *src[5] = {c->pos, c->pos0, c->frc, c->mem, c->vel};
+    for (int q = 0; q < 5; ++q)
+        CK(hipMemcpyAsync(d + q * n3, src[q], sizeof(double) * n3, hipMemcpyDeviceToDevice,
+                          c->stream));
+    CK(hipMemcpyAsync(ip, c->id, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    CK(hipMemcpyAsync(ip + c->n, c->cstart, sizeof(int) * (c->ncell + 1),
+                      hipMemcpyDeviceToDevice, c->stream));
+    A->meta[k] = {c->time, c->vel_counter, c->step_index};
+    return 0;
+}
+
+int restore_cp(psim_adjoint *A, int k)
+{
+    psim_ctx *c = A->c;
+    const size_t n3 = 3 * (size_t)c->n;
+    const double *d = A->cp_d + (size_t)k * cp_stride_d(c);
+    const int *ip = A->cp_i + (size_t)k * cp_stride_i(c);
+    double *dst[5] = {c->pos, c->pos0, c->frc, c->mem, c->vel};
+    for (int q = 0; q < 5; ++q)
+        CK(hipMemcpyAsync(dst[q], d + q * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice,
+                          c->stream));
+    CK(hipMemcpyAsync(c->id, ip, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    CK(hipMemcpyAsync(c->cstart, ip + c->n, sizeof(int) * (c->ncell + 1),
+                      hipMemcpyDeviceToDevice, c->stream));
+    c->time = A->meta[k].time;
+    c->vel_counter = A->meta[k].vel_counter;
+    c->step_index = A->meta[k].step;
+    return 0;
+}
+
+// The displacement tracker counts crossings of the trajectory that is
+// actually advanced; the adjoint's replays are not, so it is paused.
+struct TrackerPause {
+    psim_ctx *c;
+    int *saved;
+    explicit TrackerPause(psim_ctx *cc) : c(cc), saved(cc->pbc) { c->pbc = nullptr; }
+    ~TrackerPause() { c->pbc = saved; }
+};
+
+bool same_forces(const psim_params &a, const psim_params &b)
+{
+    return a.force_mask == b.force_mask && a.lj_num == b.lj_num && a.cutoff == b.cutoff &&
+           a.lj_well_depth == b.lj_well_depth && a.yukawa_strength == b.yukawa_strength &&
+           a.debye_length == b.debye_length && a.ao_well_depth == b.ao_well_depth;
+}
+
+template <int MASK>
+void launch_adj_mask(psim_adjoint *A, const AdjPot &q, const double *pos, const int *id,
+                     const int *cs, const double *lam, const double *a1, const double *a2,
+                     double *a0, double *lam_prev, double beta, double alpha_prev)
+{
+    psim_ctx *c = A->c;
+    const unsigned nb = blocks_for(c->n, kAdjBlock);
+    if (c->geo.exact_mi)
+        k_adj_step<MASK, true><<<nb, kAdjBlock, 0, c->stream>>>(
+            c->geo, c->pot, q, c->prm.radius, pos, id, cs, lam, a1, a2, a0, lam_prev, beta,
+            c->gb.coeff0, alpha_prev, c->gb.c2, A->tpart);
+    else
+        k_adj_step<MASK, false><<<nb, kAdjBlock, 0, c->stream>>>(
+            c->geo, c->pot, q, c->prm.radius, pos, id, cs, lam, a1, a2, a0, lam_prev, beta,
+            c->gb.coeff0, alpha_prev, c->gb.c2, A->tpart);
+    k_theta_accum<<<1, 64, 0, c->stream>>>(A->tpart, (int)nb, A->acc);
+}
+
+}  // namespace
+
+extern "C" {
+
+int psim_adj_create(psim_ctx *c, int nsteps, int checkpoint_every, psim_adjoint **out)
+{
+    if (!c || !out || !c->have_state || nsteps <= 0 || checkpoint_every <= 0) return PSIM_E_ARG;
+    *out = nullptr;
+    psim_adjoint *A = new (std::nothrow) psim_adjoint();
+    if (!A) return PSIM_E_NOMEM;
+    A->c = c;
+    A->K = nsteps;
+    A->every = checkpoint_every < nsteps ? checkpoint_every : nsteps;
+    A->ncp = (nsteps + A->every - 1) / A->every;
+    A->meta.resize(A->ncp + 1);
+    A->first.assign(nsteps, 0);
+    const size_t n3 = 3 * (size_t)c->n;
+    const unsigned nb = blocks_for(c->n, kAdjBlock);
+    int rc = 0;
+    if (!rc) rc = dalloc(&A->cp_d, (size_t)(A->ncp + 1) * cp_stride_d(c));
+    if (!rc) rc = dalloc(&A->cp_i, (size_t)(A->ncp + 1) * cp_stride_i(c));
+    if (!rc) rc = dalloc(&A->rec_pos, (size_t)A->every * n3);
+    if (!rc) rc = dalloc(&A->rec_i, (size_t)A->every * cp_stride_i(c));
+    for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(&A->a[k], n3);
+    for (int k = 0; k < 2 && !rc; ++k) rc = dalloc(&A->lam[k], n3);
+    if (!rc) rc = dalloc(&A->tpart, (size_t)nb * kNTheta);
+    if (!rc) rc = dalloc(&A->acc, kNTheta);
+    if (!rc) rc = save_cp(A, 0);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = PSIM_E_HIP;
+    if (rc) {
+        psim_adj_destroy(A);
+        return rc;
+    }
+    *out = A;
+    return 0;
+}
+
+void psim_adj_destroy(psim_adjoint *A)
+{
+    if (!A) return;
+    if (A->c && A->c->stream) (void)hipStreamSynchronize(A->c->stream);
+    void *bufs[] = {A->cp_d, A->cp_i, A->rec_pos, A->rec_i, A->a[0], A->a[1], A->a[2],
+                    A->lam[0], A->lam[1], A->tpart, A->acc};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    delete A;
+}
+
+int psim_adj_set_initial(psim_adjoint *A)
+{
+    if (!A) return PSIM_E_ARG;
+    int rc = save_cp(A, 0);
+    if (rc) return rc;
+    CK(hipStreamSynchronize(A->c->stream));
+    A->have_forward = false;
+    return 0;
+}
+
+int psim_adj_forward(psim_adjoint *A)
+{
+    if (!A) return PSIM_E_ARG;
+    psim_ctx *c = A->c;
+    TrackerPause pause(c);
+    int rc = restore_cp(A, 0);
+    if (rc) return rc;
+    double t = c->time;
+    for (int n = 0; n < A->K; ++n) {
+        A->first[n] = (t == 0) ? 1 : 0;   // brownianIntegrator.cpp:183
+        t += c->prm.dt;
+    }
+    for (int k = 0; k < A->ncp; ++k) {
+        if (k > 0 && (rc = save_cp(A, k))) return rc;
+        const int n0 = k * A->every;
+        const int n1 = n0 + A->every < A->K ? n0 + A->every : A->K;
+        if ((rc = eng::run_steps(c, n1 - n0, nullptr))) return rc;
+    }
+    if ((rc = save_cp(A, A->ncp))) return rc;
+    int code = 0;
+    if ((rc = eng::read_error(c, &code, nullptr, nullptr))) return rc;
+    A->prm_used = c->prm;
+    A->have_forward = code == 0;
+    return code;
+}
+
+int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double *gx0)
+{
+    if (!A || !gx || !dtheta) return PSIM_E_ARG;
+    psim_ctx *c = A->c;
+    if (!A->have_forward || !same_forces(A->prm_used, c->prm)) return PSIM_E_ARG;
+    TrackerPause pause(c);
+    const int n = c->n;
+    const size_t n3 = 3 * (size_t)n;
+    const GB &gb = c->gb;
+    const double alpha_first = gb.coeff3 * gb.dt * gb.dt * gb.m;
+    const double alpha_normal = gb.c1 + gb.c2;
+    auto alpha = [&](int step) { return A->first[step] ? alpha_first : alpha_normal; };
+    auto beta = [&](int step) { return A->first[step] ? 1.0 : gb.c0; };
+    const AdjPot q = make_adj(c->prm);
+    // a_K = gx, a_{K+1} = 0, lambda_{K-1} = alpha_{K-1} a_K
+    {
+        std::vector<double> h(n3);
+        for (int o = 0; o < n; ++o)
+            for (int d = 0; d < 3; ++d) h[(size_t)d * n + o] = gx[3 * (size_t)o + d];
+        CK(hipMemcpyAsync(A->a[1], h.data(), sizeof(double) * n3, hipMemcpyHostToDevice,
+                          c->stream));
+        CK(hipStreamSynchronize(c->stream));
+    }
+    CK(hipMemsetAsync(A->a[2], 0, sizeof(double) * n3, c->stream));
+    CK(hipMemsetAsync(A->acc, 0, sizeof(double) * kNTheta, c->stream));
+    k_scale<<<blocks_for((long long)n3, 256), 256, 0, c->stream>>>((int)n3, alpha(A->K - 1),
+                                                                   A->a[1], A->lam[0]);
+    CK(hipGetLastError());
+    double *a0 = A->a[0], *a1 = A->a[1], *a2 = A->a[2];
+    double *lam = A->lam[0], *lam_prev = A->lam[1];
+    const size_t si = cp_stride_i(c);
+    int rc = 0;
+    for (int k = A->ncp - 1; k >= 0; --k) {
+        const int n0 = k * A->every;
+        const int n1 = n0 + A->every < A->K ? n0 + A->every : A->K;
+        if ((rc = restore_cp(A, k))) return rc;
+        for (int s = n0; s < n1; ++s) {
+            const int r = s - n0;
+            CK(hipMemcpyAsync(A->rec_pos + (size_t)r * n3, c->pos, sizeof(double) * n3,
+                              hipMemcpyDeviceToDevice, c->stream));
+            CK(hipMemcpyAsync(A->rec_i + (size_t)r * si, c->id, sizeof(int) * n,
+                              hipMemcpyDeviceToDevice, c->stream));
+            CK(hipMemcpyAsync(A->rec_i + (size_t)r * si + n, c->cstart,
+                              sizeof(int) * (c->ncell + 1), hipMemcpyDeviceToDevice,
+                              c->stream));
+            if (s + 1 < n1 && (rc = eng::run_steps(c, 1, nullptr))) return rc;
+        }
+        for (int s = n1 - 1; s >= n0; --s) {
+            const int r = s - n0;
+            const double *pos = A->rec_pos + (size_t)r * n3;
+            const int *id = A->rec_i + (size_t)r * si;
+            const int *cs = id + n;
+            double *lp = s >= 1 ? lam_prev : nullptr;
+            const double ap = s >= 1 ? alpha(s - 1) : 0.0;
+            switch (c->pot.mask) {
+            case 1: launch_adj_mask<1>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
+            case 2: launch_adj_mask<2>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
+            case 3: launch_adj_mask<3>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
+            case 4: launch_adj_mask<4>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
+            case 5: launch_adj_mask<5>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
+            case 6: launch_adj_mask<6>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
+            case 7: launch_adj_mask<7>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
+            default: return PSIM_E_ARG;
+            }
+            CK(hipGetLastError());
+            // rotate: a_{n+2} <- a_{n+1}, a_{n+1} <- a_n; lambda_n <- lambda_{n-1}
+            double *t = a2;
+            a2 = a1;
+            a1 = a0;
+            a0 = t;
+            t = lam;
+            lam = lam_prev;
+            lam_prev = t;
+        }
+    }
+    // a1 now holds a_0 = dL/dx_0
+    std::vector<double> h(gx0 ? n3 : 0);
+    if (gx0)
+        CK(hipMemcpyAsync(h.data(), a1, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
+    CK(hipMemcpyAsync(dtheta, A->acc, sizeof(double) * kNTheta, hipMemcpyDeviceToHost,
+                      c->stream));
+    // leave the context at the end of the trajectory
+    if ((rc = restore_cp(A, A->ncp))) return rc;
+    CK(hipStreamSynchronize(c->stream));
+    if (gx0)
+        for (int o = 0; o < n; ++o)
+            for (int d = 0; d < 3; ++d) gx0[3 * (size_t)o + d] = h[(size_t)d * n + o];
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,570 @@
+// psim_analysis.hip -- on-device analysis of a context's current state:
+// kinetic temperature, mean displacement, the tracked mean-square
+// displacement, contact counts (coordination), contact clusters, and a
+// Gaussian-smoothed pair-distance histogram with its position gradient.
+//
+// Definitions follow the reference's analysis manager:
+//   temperature        analysisManager::getTemperature (analysisGeneral.cpp:26-44)
+//   mean_disp2         analysisManager::meanDisplacement (analysisGeneral.cpp:46-56)
+//   msd                analysisManager::trackedDisplacement
+//                      (analysisTracker.cpp:26-85; FP64 here, float there)
+//   contacts           system::updateInteractions / iterateParticleInteractions
+//                      (systemHandling.cpp:94-161): r < contact radius over
+//                      the 27 cells of the force grid
+//   clusters           analysisManager::findClusters / writeClusters
+//                      (analysisCluster.cpp:26-100)
+// The pair histogram has no reference counterpart (the reference writes no
+// g(r)); it is the observable of the structure-fitting loss.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "psim_build.hpp"
+#include "psim_ctx.hpp"
+
+using namespace psim;
+using eng::blocks_for;
+using eng::dalloc;
+
+namespace {
+
+constexpr int kAnaBlock = 256;
+constexpr int kMaxBins = 2048;        // bins held in LDS by the histogram kernel
+constexpr double kWindow = 8.0;       // Gaussian support in units of sigma
+constexpr int kMaxHistCells = 256;    // cells per dimension of the histogram grid
+
+// ---- block reductions (256 lanes = 4 waves) ---------------------------------
+template <int NQ>
+__device__ void block_sums(double (&v)[NQ], double *out)
+{
+    __shared__ double red[NQ][kAnaBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double x = v[q];
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) x += __shfl_down(x, d, 64);
+        if (lane == 0) red[q][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        double t = 0.0;
+        for (int k = 0; k < kAnaBlock / 64; ++k) t += red[threadIdx.x][k];
+        out[(size_t)blockIdx.x * NQ + threadIdx.x] = t;
+    }
+}
+
+// ---- displacement tracker ---------------------------------------------------
+// analysisManager::initDisplacementTracker (analysisTracker.cpp:26-36)
+__global__ void k_track_init(int n, const double *pos, const int *id, double *track0, int *pbc)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int o = id[s];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        track0[(size_t)d * n + o] = pos[(size_t)d * n + s];
+        pbc[(size_t)d * n + o] = 0;
+    }
+}
+
+// Contacts of slot s: the 27 cells of the force grid around the particle's
+// own cell, r = sqrt(pbcDist) < rc (systemHandling.cpp:94-122, 124-161).
+template <class Fn>
+__device__ void visit_contacts(const Geo &g, const double *pos, const int *cstart, int s,
+                               double rc, Fn fn)
+{
+    const int N = g.n, S = g.S;
+    const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
+    const int cx = cell_1d(xi, g.cell, S), cy = cell_1d(yi, g.cell, S),
+              cz = cell_1d(zi, g.cell, S);
+    for (int ox = -1; ox <= 1; ++ox)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int oz = -1; oz <= 1; ++oz) {
+                int rx = (cx + ox) % S, ry = (cy + oy) % S, rz = (cz + oz) % S;
+                rx = rx < 0 ? rx + S : rx;
+                ry = ry < 0 ? ry + S : ry;
+                rz = rz < 0 ? rz + S : rz;
+                const int h = rx + S * ry + g.S2 * rz;
+                for (int j = cstart[h]; j < cstart[h + 1]; ++j) {
+                    if (j == s) continue;
+                    const double r2 = pbc_dist_ref(xi, yi, zi, pos[j], pos[N + j],
+                                                   pos[2 * N + j], g.L, g.half_pbc);
+                    if (sqrt(r2) < rc) fn(j);
+                }
+            }
+}
+
+// One pass: per-particle coordination (by original index) and per-block
+// partial sums of v^2, |x - x0|^2, the tracked squared displacement and the
+// contact count.
+__global__ __launch_bounds__(kAnaBlock) void k_observables(
+    Geo g, const double *pos, const double *pos0, const double *vel, const int *id,
+    const int *cstart, const double *track0, const int *pbc, double contact_r, int *coord,
+    double *partials)
+{
+    const int s = blockIdx.x * kAnaBlock + threadIdx.x;
+    const int N = g.n;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (s < N) {
+        const int o = id[s];
+        // getTemperature: vtot += vx*vx; += vy*vy; += vz*vz
+        double vt = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double vd = vel[(size_t)d * N + o];
+            vt += (vd * vd);
+        }
+        v[0] = vt;
+        // meanDisplacement: dx*dx + dy*dy + dz*dz with dx = x - x0
+        const double dx = pos[s] - pos0[s], dy = pos[N + s] - pos0[N + s],
+                     dz = pos[2 * N + s] - pos0[2 * N + s];
+        v[1] = dx * dx + dy * dy + dz * dz;
+        if (track0) {
+            // trackedDisplacement: dx = x - xStart; dx += boxSize * xPBC
+            double m = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double e = pos[(size_t)d * N + s] - track0[(size_t)d * N + o];
+                e += g.L * (double)pbc[(size_t)d * N + o];
+                m += (e * e);
+            }
+            v[2] = m;
+        }
+        int nc = 0;
+        if (contact_r > 0.0) visit_contacts(g, pos, cstart, s, contact_r, [&](int) { ++nc; });
+        if (coord) coord[o] = nc;
+        v[3] = (double)nc;
+    }
+    block_sums<4>(v, partials);
+}
+
+// ---- contact clusters (connected components) -------------------------------
+__global__ void k_cc_init(int n, int *lab)
+{
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o < n) lab[o] = o;
+}
+
+// Minimum-label propagation over the contact graph.  Labels are always the
+// original index of a member of the same component and never increase.
+__global__ __launch_bounds__(kAnaBlock) void k_cc_hook(Geo g, const double *pos,
+                                                       const int *id, const int *cstart,
+                                                       double rc, int *lab, int *changed)
+{
+    const int s = blockIdx.x * kAnaBlock + threadIdx.x;
+    if (s >= g.n) return;
+    const int o = id[s];
+    int m = lab[o];
+    visit_contacts(g, pos, cstart, s, rc, [&](int j) { m = min(m, lab[id[j]]); });
+    if (m < lab[o]) {
+        atomicMin(&lab[o], m);
+        atomicMin(&lab[lab[o]], m);
+        changed[0] = 1;
+    }
+}
+
+// Pointer jumping: lab[o] <- root of its label chain (lab[x] <= x, so every
+// chain strictly decreases until it reaches a fixed point).
+__global__ void k_cc_jump(int n, int *lab)
+{
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    int l = lab[o];
+    for (int k = 0; k < n; ++k) {
+        const int nl = lab[l];
+        if (nl == l) break;
+        l = nl;
+    }
+    lab[o] = l;
+}
+
+// ---- pair-distance histogram -----------------------------------------------
+struct HistGeo {
+    int n;
+    int S, S2;          // histogram grid (S >= 3), or S == 0 for all pairs
+    double L, half, cell;
+    double r_lo, width, sigma, inv_sigma, reach2;
+    int nbins;
+};
+
+__device__ inline double norm_cdf(double t) { return 0.5 * erfc(-t * 0.70710678118654752440); }
+__device__ inline double norm_pdf(double t) { return 0.39894228040143267794 * exp(-0.5 * t * t); }
+
+// visit every j != s within reach of slot s (histogram grid, or all pairs)
+template <class Fn>
+__device__ void visit_reach(const HistGeo &h, const double *pos, const int *perm,
+                            const int *cstart, int s, Fn fn)
+{
+    const int N = h.n;
+    const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
+    auto test = [&](int j) {
+        if (j == s) return;
+        const double xj = pos[j], yj = pos[N + j], zj = pos[2 * N + j];
+        const double r2 = pbc_dist_ref(xi, yi, zi, xj, yj, zj, h.L, h.half);
+        if (r2 < h.reach2) fn(j, xj, yj, zj, r2);
+    };
+    if (h.S == 0) {
+        for (int j = 0; j < N; ++j) test(j);
+        return;
+    }
+    const int S = h.S;
+    const int cx = cell_1d(xi, h.cell, S), cy = cell_1d(yi, h.cell, S),
+              cz = cell_1d(zi, h.cell, S);
+    for (int ox = -1; ox <= 1; ++ox)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int oz = -1; oz <= 1; ++oz) {
+                int rx = cx + ox, ry = cy + oy, rz = cz + oz;
+                rx = rx < 0 ? rx + S : (rx >= S ? rx - S : rx);
+                ry = ry < 0 ? ry + S : (ry >= S ? ry - S : ry);
+                rz = rz < 0 ? rz + S : (rz >= S ? rz - S : rz);
+                const int c = rx + S * ry + h.S2 * rz;
+                for (int u = cstart[c]; u < cstart[c + 1]; ++u) test(perm[u]);
+            }
+}
+
+// edge window of a pair at distance r: edges k in [k0, k1] carry the
+// Gaussian's mass; below k0 the CDF is 0, above k1 it is 1 (to 6e-16)
+__device__ inline void edge_window(const HistGeo &h, double r, int &k0, int &k1)
+{
+    const double lo = (r - kWindow * h.sigma - h.r_lo) / h.width;
+    const double hi = (r + kWindow * h.sigma - h.r_lo) / h.width;
+    k0 = (int)fmax(0.0, ceil(lo));
+    k1 = (int)fmin((double)h.nbins, floor(hi));
+}
+
+// h_b = sum over ordered pairs i != j of the Gaussian mass of r_ij in bin b
+// (sigma > 0), or the count of r_ij in [e_b, e_b+1) (sigma == 0).
+__global__ __launch_bounds__(kAnaBlock) void k_pair_hist(HistGeo h, const double *pos,
+                                                         const int *perm, const int *cstart,
+                                                         double *hist)
+{
+    __shared__ double sh[kMaxBins];
+    for (int b = threadIdx.x; b < h.nbins; b += kAnaBlock) sh[b] = 0.0;
+    __syncthreads();
+    const int t = blockIdx.x * kAnaBlock + threadIdx.x;
+    if (t < h.n) {
+        const int s = h.S ? perm[t] : t;
+        visit_reach(h, pos, perm, cstart, s, [&](int, double, double, double, double r2) {
+            const double r = sqrt(r2);
+            if (h.sigma == 0.0) {
+                const double q = floor((r - h.r_lo) / h.width);
+                if (q >= 0.0 && q < (double)h.nbins) atomicAdd(&sh[(int)q], 1.0);
+                return;
+            }
+            int k0, k1;
+            edge_window(h, r, k0, k1);
+            double prev = 0.0;   // CDF at the edge below
+            for (int k = k0; k <= k1; ++k) {
+                const double e = h.r_lo + h.width * k;
+                const double E = norm_cdf((e - r) * h.inv_sigma);
+                if (k - 1 >= 0 && k - 1 < h.nbins) atomicAdd(&sh[k - 1], E - prev);
+                prev = E;
+            }
+            if (k1 < h.nbins && k1 >= 0 && k0 <= k1) atomicAdd(&sh[k1], 1.0 - prev);
+            else if (k0 > k1 && k1 >= 0 && k1 < h.nbins) atomicAdd(&sh[k1], 1.0);
+        });
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < h.nbins; b += kAnaBlock)
+        if (sh[b] != 0.0) atomicAdd(&hist[b], sh[b]);
+}
+
+// grad_i = d(sum_b w_b h_b)/dx_i = -2 sum_j S(r_ij) d_ij / r_ij with
+// S(r) = sum_b w_b dm_b/dr and d_ij the minimum-image x_j - x_i.
+__global__ __launch_bounds__(kAnaBlock) void k_pair_hist_grad(HistGeo h, const double *pos,
+                                                              const int *perm,
+                                                              const int *cstart,
+                                                              const int *id, const double *w,
+                                                              double *grad)
+{
+    const int t = blockIdx.x * kAnaBlock + threadIdx.x;
+    if (t >= h.n) return;
+    const int N = h.n;
+    const int s = h.S ? perm[t] : t;
+    const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    visit_reach(h, pos, perm, cstart, s, [&](int, double xj, double yj, double zj, double r2) {
+        const double r = sqrt(r2);
+        int k0, k1;
+        edge_window(h, r, k0, k1);
+        double sum = 0.0;
+        for (int k = k0; k <= k1; ++k) {
+            const double e = h.r_lo + h.width * k;
+            const double wk = (k < h.nbins) ? w[k] : 0.0;
+            const double wm = (k >= 1) ? w[k - 1] : 0.0;
+            sum += norm_pdf((e - r) * h.inv_sigma) * (wk - wm);
+        }
+        const double S = sum * h.inv_sigma;
+        double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+        dx = dx > h.half ? dx - h.L : (dx < -h.half ? dx + h.L : dx);
+        dy = dy > h.half ? dy - h.L : (dy < -h.half ? dy + h.L : dy);
+        dz = dz > h.half ? dz - h.L : (dz < -h.half ? dz + h.L : dz);
+        const double c = -2.0 * S / r;
+        gx += c * dx;
+        gy += c * dy;
+        gz += c * dz;
+    });
+    const int o = id[s];
+    grad[o] = gx;
+    grad[N + o] = gy;
+    grad[2 * N + o] = gz;
+}
+
+// canonical within-cell order of the histogram grid (ascending slot), so the
+// neighbour visit order does not depend on atomic timing
+__global__ void k_hg_canon(int n, const int *perm_in, const int *key, const int *cstart,
+                           int *perm_out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int s = perm_in[t];
+    const int c = key[s];
+    const int b = cstart[c], e = cstart[c + 1];
+    int rank = 0;
+    for (int u = b; u < e; ++u) rank += (perm_in[u] < s);
+    perm_out[b + rank] = s;
+}
+
+// ---- host helpers -------------------------------------------------------------
+int ensure_work(psim_ctx *c, size_t doubles, size_t ints)
+{
+    if (doubles > c->work_cap) {
+        if (c->work) CK(hipFree(c->work));
+        c->work = nullptr;
+        c->work_cap = 0;
+        int rc = dalloc(&c->work, doubles);
+        if (rc) return rc;
+        c->work_cap = doubles;
+    }
+    if (ints > c->iwork_cap) {
+        if (c->iwork) CK(hipFree(c->iwork));
+        c->iwork = nullptr;
+        c->iwork_cap = 0;
+        int rc = dalloc(&c->iwork, ints);
+        if (rc) return rc;
+        c->iwork_cap = ints;
+    }
+    return 0;
+}
+
+int check_hist_args(psim_ctx *c, double r_lo, double width, int nbins, double sigma)
+{
+    if (!(r_lo >= 0.0) || !(width > 0.0) || nbins <= 0 || nbins > kMaxBins || !(sigma >= 0.0))
+        return PSIM_E_ARG;
+    const double reach = r_lo + width * nbins + kWindow * sigma;
+    // pairs beyond half the box have no unique minimum image
+    if (reach > 0.5 * c->geo.L) return PSIM_E_UNSUPPORTED;
+    return 0;
+}
+
+// Build the histogram grid for the current positions; fills h.
+int build_hist_grid(psim_ctx *c, double r_lo, double width, int nbins, double sigma,
+                    HistGeo &h)
+{
+    const int n = c->n;
+    const double L = c->geo.L;
+    const double reach = r_lo + width * nbins + kWindow * sigma;
+    h.n = n;
+    h.L = L;
+    h.half = c->geo.half_pbc;
+    h.r_lo = r_lo;
+    h.width = width;
+    h.sigma = sigma;
+    h.inv_sigma = sigma > 0.0 ? 1.0 / sigma : 0.0;
+    h.reach2 = reach * reach;
+    h.nbins = nbins;
+    int S = (int)std::floor(L / reach);
+    if (S > kMaxHistCells) S = kMaxHistCells;
+    if (S < 3) {
+        h.S = h.S2 = 0;
+        h.cell = L;
+        return 0;
+    }
+    h.S = S;
+    h.S2 = S * S;
+    h.cell = L / S;
+    auto &g = c->hg;
+    const int ncell = S * S * S;
+    if ((size_t)ncell > g.cap_cells || !g.key) {
+        void *bufs[] = {g.key, g.rank, g.perm, g.count, g.cstart, g.bsum};
+        for (void *b : bufs)
+            if (b) CK(hipFree(b));
+        g = psim_ctx::HistGrid{};
+        const int nscan = (ncell + kScanTile - 1) / kScanTile;
+        int rc = 0;
+        if (!rc) rc = dalloc(&g.key, n);
+        if (!rc) rc = dalloc(&g.rank, n);
+        if (!rc) rc = dalloc(&g.perm, 2 * (size_t)n);
+        if (!rc) rc = dalloc(&g.count, ncell);
+        if (!rc) rc = dalloc(&g.cstart, (size_t)ncell + 1);
+        if (!rc) rc = dalloc(&g.bsum, nscan);
+        if (rc) return rc;
+        CK(hipMemsetAsync(g.count, 0, sizeof(int) * ncell, c->stream));
+        g.cap_cells = ncell;
+    }
+    g.S = S;
+    g.ncell = ncell;
+    g.nscan = (ncell + kScanTile - 1) / kScanTile;
+    g.cell = h.cell;
+    Geo hg = c->geo;
+    hg.S = S;
+    hg.S2 = S * S;
+    hg.ncell = ncell;
+    hg.cell = h.cell;
+    hg.c0 = 0;
+    hipStream_t st = c->stream;
+    k_hash_count<<<blocks_for(n, 256), 256, 0, st>>>(hg, c->pos, g.key, g.rank, g.count);
+    k_scan_reduce<<<g.nscan, kScanThreads, 0, st>>>(g.count, ncell, g.bsum);
+    k_scan_bsums<<<1, kScanThreads, 0, st>>>(g.bsum, g.nscan);
+    k_scan_apply<<<g.nscan, kScanThreads, 0, st>>>(g.count, ncell, g.bsum, g.cstart, n);
+    k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, g.key, g.rank, g.cstart, g.perm + n);
+    k_hg_canon<<<blocks_for(n, 256), 256, 0, st>>>(n, g.perm + n, g.key, g.cstart, g.perm);
+    CK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psim_gpu_track_start(psim_ctx *c)
+{
+    if (!c || !c->have_state) return PSIM_E_ARG;
+    const int n = c->n;
+    if (!c->pbc) {
+        int rc = dalloc(&c->pbc, 3 * (size_t)n);
+        if (!rc) rc = dalloc(&c->track0, 3 * (size_t)n);
+        if (rc) return rc;
+    }
+    k_track_init<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, c->pos, c->id, c->track0, c->pbc);
+    CK(hipGetLastError());
+    CK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int psim_gpu_observables(psim_ctx *c, double contact_r, psim_observables *out,
+                         int *coord_by_orig)
+{
+    if (!c || !out || !c->have_state || !(contact_r >= 0.0)) return PSIM_E_ARG;
+    const int n = c->n;
+    const unsigned nb = blocks_for(n, kAnaBlock);
+    int rc = ensure_work(c, (size_t)nb * 4, (size_t)n);
+    if (rc) return rc;
+    k_observables<<<nb, kAnaBlock, 0, c->stream>>>(c->geo, c->pos, c->pos0, c->vel, c->id,
+                                                   c->cstart, c->track0, c->pbc, contact_r,
+                                                   c->iwork, c->work);
+    CK(hipGetLastError());
+    std::vector<double> part((size_t)nb * 4);
+    CK(hipMemcpyAsync(part.data(), c->work, sizeof(double) * part.size(),
+                      hipMemcpyDeviceToHost, c->stream));
+    if (coord_by_orig)
+        CK(hipMemcpyAsync(coord_by_orig, c->iwork, sizeof(int) * n, hipMemcpyDeviceToHost,
+                          c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    for (unsigned b = 0; b < nb; ++b)
+        for (int q = 0; q < 4; ++q) sum[q] += part[(size_t)b * 4 + q];
+    out->temperature = sum[0] / 3.0 / n;   // vtot / 3.0 / nParticles
+    out->mean_disp2 = sum[1] / (double)n;
+    out->msd = c->track0 ? sum[2] / n : std::nan("");
+    out->contacts = (long long)std::llround(sum[3]);
+    out->mean_coordination = (double)out->contacts / (double)n;
+    out->time = c->time;
+    out->step = c->step_index;
+    return 0;
+}
+
+int psim_gpu_clusters(psim_ctx *c, double contact_r, int min_size, int *labels_by_orig,
+                      int *n_clusters, int *mean_size)
+{
+    if (!c || !c->have_state || !(contact_r > 0.0)) return PSIM_E_ARG;
+    const int n = c->n;
+    const unsigned nb = blocks_for(n, kAnaBlock);
+    int rc = ensure_work(c, 1, (size_t)n + 1);
+    if (rc) return rc;
+    int *lab = c->iwork, *changed = c->iwork + n;
+    k_cc_init<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, lab);
+    CK(hipGetLastError());
+    // labels only decrease and each round lowers at least one: at most n rounds
+    for (int it = 0; it <= n; ++it) {
+        CK(hipMemsetAsync(changed, 0, sizeof(int), c->stream));
+        k_cc_hook<<<nb, kAnaBlock, 0, c->stream>>>(c->geo, c->pos, c->id, c->cstart, contact_r,
+                                                   lab, changed);
+        k_cc_jump<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, lab);
+        CK(hipGetLastError());
+        int hc = 0;
+        CK(hipMemcpyAsync(&hc, changed, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        CK(hipStreamSynchronize(c->stream));
+        if (!hc) break;
+    }
+    std::vector<int> lh(n);
+    CK(hipMemcpyAsync(lh.data(), lab, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    if (labels_by_orig) std::memcpy(labels_by_orig, lh.data(), sizeof(int) * n);
+    // writeClusters (analysisCluster.cpp:86-100): clusters of more than
+    // min_size - 1 members, average size by integer division
+    std::vector<int> size(n, 0);
+    for (int o = 0; o < n; ++o) size[lh[o]]++;
+    long long total = 0;
+    int count = 0;
+    for (int o = 0; o < n; ++o)
+        if (size[o] >= min_size && size[o] > 0) {
+            total += size[o];
+            ++count;
+        }
+    if (n_clusters) *n_clusters = count;
+    if (mean_size) *mean_size = count ? (int)(total / count) : 0;
+    return 0;
+}
+
+int psim_gpu_pair_histogram(psim_ctx *c, double r_lo, double width, int nbins, double sigma,
+                            double *hist)
+{
+    if (!c || !hist || !c->have_state) return PSIM_E_ARG;
+    int rc = check_hist_args(c, r_lo, width, nbins, sigma);
+    if (rc) return rc;
+    HistGeo h{};
+    rc = build_hist_grid(c, r_lo, width, nbins, sigma, h);
+    if (rc) return rc;
+    rc = ensure_work(c, (size_t)nbins, 1);
+    if (rc) return rc;
+    CK(hipMemsetAsync(c->work, 0, sizeof(double) * nbins, c->stream));
+    k_pair_hist<<<blocks_for(c->n, kAnaBlock), kAnaBlock, 0, c->stream>>>(
+        h, c->pos, c->hg.perm, c->hg.cstart, c->work);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(hist, c->work, sizeof(double) * nbins, hipMemcpyDeviceToHost,
+                      c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int psim_gpu_pair_histogram_grad(psim_ctx *c, double r_lo, double width, int nbins,
+                                 double sigma, const double *w, double *grad)
+{
+    if (!c || !w || !grad || !c->have_state || !(sigma > 0.0)) return PSIM_E_ARG;
+    int rc = check_hist_args(c, r_lo, width, nbins, sigma);
+    if (rc) return rc;
+    HistGeo h{};
+    rc = build_hist_grid(c, r_lo, width, nbins, sigma, h);
+    if (rc) return rc;
+    const int n = c->n;
+    rc = ensure_work(c, (size_t)nbins + 3 * (size_t)n, 1);
+    if (rc) return rc;
+    double *dw = c->work, *dg = c->work + nbins;
+    CK(hipMemcpyAsync(dw, w, sizeof(double) * nbins, hipMemcpyHostToDevice, c->stream));
+    k_pair_hist_grad<<<blocks_for(n, kAnaBlock), kAnaBlock, 0, c->stream>>>(
+        h, c->pos, c->hg.perm, c->hg.cstart, c->id, dw, dg);
+    CK(hipGetLastError());
+    std::vector<double> hg(3 * (size_t)n);
+    CK(hipMemcpyAsync(hg.data(), dg, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                      c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    for (int o = 0; o < n; ++o)
+        for (int d = 0; d < 3; ++d) grad[3 * (size_t)o + d] = hg[(size_t)d * n + o];
+    return 0;
+}
+
+}  // extern "C"

@@ -29,7 +29,7 @@
 #include <new>
 #include <vector>
 
-#include "psim_gpu.h"
+#include "psim_ctx.hpp"
 #include "psim_build.hpp"
 #include "psim_params.hpp"
 #include "psim_step.hpp"
@@ -37,72 +37,12 @@
 using namespace psim;
 
 
-// ---------------------------------------------------------------------------
-// Context
-// ---------------------------------------------------------------------------
-struct psim_ctx {
-    psim_params prm;
-    Geo geo;
-    Pot pot;
-    GB gb;
-    int n = 0, ncell = 0, nscan = 0;
-    bool dense = false;   // cell-per-workgroup pair kernel (else thread per particle)
-    hipStream_t stream = nullptr;
-    double *pos = nullptr, *pos0 = nullptr, *frc = nullptr, *mem = nullptr;
-    double *npos = nullptr, *npos0 = nullptr, *nfrc = nullptr, *nmem = nullptr;
-    double *vel = nullptr, *fout = nullptr;
-    int *id = nullptr, *nid = nullptr, *key = nullptr, *arank = nullptr, *perm = nullptr;
-    int *count = nullptr, *cstart = nullptr, *bsum = nullptr;
-    DevErr *err = nullptr;
-    double *noise = nullptr;
-    size_t noise_cap = 0;
-    double *tsorted = nullptr;
-    int *thi = nullptr, *tcse = nullptr;
-    double time = 0.0;
-    int vel_counter = 0;
-    unsigned long long step_index = 0;
-    bool have_state = false;
-    // profiling
-    bool profiling = false;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used = 0;
-    struct Mark { size_t e0, e1, e1b, e2; };
-    std::vector<Mark> marks;
-    psim_kernel_times times{};
-};
-
-#define CK(x)                                                                    \
-    do {                                                                         \
-        hipError_t _e = (x);                                                     \
-        if (_e != hipSuccess) {                                                  \
-            std::fprintf(stderr, "psim_gpu: %s failed: %s (%s:%d)\n", #x,        \
-                         hipGetErrorString(_e), __FILE__, __LINE__);             \
-            return PSIM_E_HIP;                                                   \
-        }                                                                        \
-    } while (0)
-
 namespace {
 
+using eng::blocks_for;
+using eng::dalloc;
+
 void fill_constants(psim_ctx *c) { make_constants(c->prm, c->geo, c->pot, c->gb); }
-
-template <class T>
-int dalloc(T **p, size_t count)
-{
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
-    if (e != hipSuccess) {
-        std::fprintf(stderr, "psim_gpu: hipMalloc(%zu B) failed: %s\n", count * sizeof(T),
-                     hipGetErrorString(e));
-        *p = nullptr;
-        return PSIM_E_NOMEM;
-    }
-    return 0;
-}
-
-inline unsigned blocks_for(long long n, int threads)
-{
-    return (unsigned)((n + threads - 1) / threads);
-}
 
 hipEvent_t take_event(psim_ctx *c, size_t *idx)
 {
@@ -208,12 +148,13 @@ int run_steps(psim_ctx *c, int nsteps, const double *d_noise)
     if (!c->have_state) return PSIM_E_ARG;
     const int n = c->n;
     for (int s = 0; s < nsteps; ++s) {
-        StepArgs a;
+        StepArgs a{};
         a.pos = c->pos; a.pos0 = c->pos0; a.frc = c->frc; a.mem = c->mem;
         a.id = c->id; a.cstart = c->cstart;
         a.npos = c->npos; a.npos0 = c->npos0; a.nfrc = c->nfrc; a.nmem = c->nmem;
         a.nid = c->nid; a.vel = c->vel; a.key = c->key; a.arank = c->arank;
         a.count = c->count; a.err = c->err;
+        a.pbc = c->pbc;
         a.noise = d_noise ? d_noise + (size_t)s * 6 * n : nullptr;
         a.seed = c->prm.seed;
         a.step = c->step_index;
@@ -273,6 +214,31 @@ int read_error(psim_ctx *c, int *code, int *i, int *j)
 }
 
 }  // namespace
+
+namespace psim {
+namespace eng {
+
+int dalloc_bytes(void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "psim_gpu: hipMalloc(%zu B) failed: %s\n", bytes,
+                     hipGetErrorString(e));
+        *p = nullptr;
+        return PSIM_E_NOMEM;
+    }
+    return 0;
+}
+
+int launch_build(psim_ctx *c) { return ::launch_build(c); }
+int run_steps(psim_ctx *c, int nsteps, const double *d_noise)
+{
+    return ::run_steps(c, nsteps, d_noise);
+}
+int read_error(psim_ctx *c, int *code, int *i, int *j) { return ::read_error(c, code, i, j); }
+
+}  // namespace eng
+}  // namespace psim
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -368,7 +334,9 @@ void psim_gpu_destroy(psim_ctx *c)
     void *bufs[] = {c->pos, c->pos0, c->frc, c->mem, c->npos, c->npos0, c->nfrc,
                     c->nmem, c->vel, c->fout, c->id, c->nid, c->key, c->arank,
                     c->perm, c->count, c->cstart, c->bsum, c->err, c->noise,
-                    c->tsorted, c->thi, c->tcse};
+                    c->tsorted, c->thi, c->tcse, c->pbc, c->track0, c->hg.key,
+                    c->hg.rank, c->hg.perm, c->hg.count, c->hg.cstart, c->hg.bsum,
+                    c->work, c->iwork};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -635,5 +603,23 @@ int psim_gpu_kernel_times(psim_ctx *c, psim_kernel_times *out, int reset)
 }
 
 void *psim_gpu_stream(psim_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int psim_gpu_set_force_params(psim_ctx *c, const psim_params *params)
+{
+    if (!c || !params) return PSIM_E_ARG;
+    const psim_params &P = *params;
+    if (!(P.cutoff > 0) || P.force_mask <= 0 || P.force_mask > 7) return PSIM_E_ARG;
+    if ((P.force_mask & PSIM_FORCE_LJ) && !(P.debye_length > 0)) return PSIM_E_ARG;
+    CK(hipStreamSynchronize(c->stream));
+    c->prm.force_mask = P.force_mask;
+    c->prm.lj_num = P.lj_num;
+    c->prm.cutoff = P.cutoff;
+    c->prm.lj_well_depth = P.lj_well_depth;
+    c->prm.yukawa_strength = P.yukawa_strength;
+    c->prm.debye_length = P.debye_length;
+    c->prm.ao_well_depth = P.ao_well_depth;
+    fill_constants(c);
+    return 0;
+}
 
 }  // extern "C"

@@ -47,6 +47,7 @@ struct StepArgs {
     unsigned long long seed, step;
     int first, do_vel;
     const StepCtl *ctl;  // when set, step / first / do_vel are read from it
+    int *pbc;            // [3][N] periodic crossings by original index, or null
 };
 
 // G&B epilogue for the particle in slot s with total force F
@@ -124,6 +125,15 @@ __device__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int
         xw[d] = safe_mod(xn[d], g.L);
         double x0n = safe_mod0(xc[d], xw[d], g.L);
         bad |= (xw[d] < 0.0) || (xw[d] >= g.L);
+        if (a.pbc) {
+            // analysisManager::updateParticlePBC (analysisTracker.cpp:38-56):
+            // the re-imaged previous position leaves the box exactly when the
+            // particle crossed a face this step
+            if (x0n < 0)
+                a.pbc[(size_t)d * N + o] += 1;
+            else if (x0n > g.L)
+                a.pbc[(size_t)d * N + o] -= 1;
+        }
         a.npos[d * N + s] = xw[d];
         a.npos0[d * N + s] = x0n;
         if (WRITE_F) a.nfrc[d * N + s] = F[d];

@@ -395,6 +395,132 @@ class GpuSystem:
     def stream(self) -> int:
         return self.lib.psim_gpu_stream(self._ctx) or 0
 
+    # -- parameters -----------------------------------------------------------
+    def set_forces(self, forces) -> None:
+        """Replace the force modules' parameters (same modules or others);
+        geometry, integrator and state are kept."""
+        if not isinstance(forces, ForceSet):
+            forces = ForceSet(list(forces) if isinstance(forces, (list, tuple)) else [forces])
+        p = PsimParams.from_buffer_copy(self.params)
+        forces.fill(p)
+        check(self.lib.psim_gpu_set_force_params(self._ctx, C.byref(p)),
+              "psim_gpu_set_force_params")
+        self.params, self.forces = p, forces
+
+    # -- analysis -------------------------------------------------------------
+    def track_start(self) -> None:
+        """Start the displacement tracker at the current positions."""
+        check(self.lib.psim_gpu_track_start(self._ctx), "psim_gpu_track_start")
+
+    def observables(self, contact_r: float = 1.2, coordination: bool = False):
+        """Temperature, mean displacement, tracked MSD and contact counts of
+        the current state (the reference's analysis definitions)."""
+        ob = _lib.Observables()
+        coord = np.empty(self.n, np.int32) if coordination else None
+        check(self.lib.psim_gpu_observables(
+            self._ctx, float(contact_r), C.byref(ob),
+            None if coord is None else coord.ctypes.data_as(C.POINTER(C.c_int))),
+            "psim_gpu_observables")
+        out = {f: getattr(ob, f) for f, _ in ob._fields_}
+        if coordination:
+            out["coordination"] = coord
+        return out
+
+    def clusters(self, contact_r: float = 1.2, min_size: int = 5):
+        """Connected components of the contact graph: (labels by original
+        index, number of clusters of >= min_size members, mean size)."""
+        lab = np.empty(self.n, np.int32)
+        nc, ms = C.c_int(), C.c_int()
+        ip = C.POINTER(C.c_int)
+        check(self.lib.psim_gpu_clusters(self._ctx, float(contact_r), int(min_size),
+                                         lab.ctypes.data_as(ip), C.byref(nc), C.byref(ms)),
+              "psim_gpu_clusters")
+        return lab, nc.value, ms.value
+
+    def pair_histogram(self, r_lo: float, width: float, nbins: int, sigma: float = 0.0):
+        """Pair-distance histogram over ordered pairs (Gaussian-smoothed when
+        sigma > 0)."""
+        out = np.empty(int(nbins))
+        check(self.lib.psim_gpu_pair_histogram(self._ctx, float(r_lo), float(width),
+                                               int(nbins), float(sigma), _ptr(out)),
+              "psim_gpu_pair_histogram")
+        return out
+
+    def pair_histogram_grad(self, r_lo: float, width: float, nbins: int, sigma: float, w):
+        """d(sum_b w_b hist_b)/d positions, AoS3 by original index."""
+        wv = np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
+        if wv.size != nbins:
+            raise ValueError("w must have nbins entries")
+        out = np.empty(3 * self.n)
+        check(self.lib.psim_gpu_pair_histogram_grad(self._ctx, float(r_lo), float(width),
+                                                    int(nbins), float(sigma), _ptr(wv),
+                                                    _ptr(out)),
+              "psim_gpu_pair_histogram_grad")
+        return out.reshape(self.n, 3)
+
+
+ADJOINT_PARAMS = ("ao_well_depth", "cutoff", "lj_well_depth", "yukawa_strength",
+                  "debye_length")
+
+
+class Adjoint:
+    """Reverse-mode derivative of ``nsteps`` steps of a GpuSystem (in-kernel
+    noise, fixed by the seed and step index) with respect to the pair-force
+    parameters, with a checkpoint every ``checkpoint_every`` steps.
+
+    The trajectory starts from the system's state when the object is made
+    (or at the last ``set_initial``)."""
+
+    def __init__(self, system: GpuSystem, nsteps: int, checkpoint_every: int = 20):
+        self.system, self.nsteps, self.every = system, int(nsteps), int(checkpoint_every)
+        self.lib = system.lib
+        self._adj = C.c_void_p()
+        check(self.lib.psim_adj_create(system._ctx, self.nsteps, self.every,
+                                       C.byref(self._adj)), "psim_adj_create")
+
+    def close(self):
+        if self._adj:
+            self.lib.psim_adj_destroy(self._adj)
+            self._adj = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_initial(self) -> None:
+        check(self.lib.psim_adj_set_initial(self._adj), "psim_adj_set_initial")
+
+    def forward(self) -> int:
+        """Run the trajectory with the system's current parameters; returns
+        the status (0 or a physics error code)."""
+        st = self.lib.psim_adj_forward(self._adj)
+        if st < 0:
+            check(st, "psim_adj_forward")
+        return st
+
+    def backward(self, gx, with_gx0: bool = False):
+        """gx = dL/dx at the last step (n x 3, original order) -> dict of
+        dL/dparameter (and dL/dx at the start)."""
+        g = np.ascontiguousarray(gx, dtype=np.float64).reshape(-1)
+        if g.size != 3 * self.system.n:
+            raise ValueError("gx must be n x 3")
+        dth = np.empty(_lib.DP_COUNT)
+        gx0 = np.empty(3 * self.system.n) if with_gx0 else None
+        check(self.lib.psim_adj_backward(self._adj, _ptr(g), _ptr(dth), _ptr(gx0)),
+              "psim_adj_backward")
+        out = dict(zip(ADJOINT_PARAMS, dth.tolist()))
+        if with_gx0:
+            out["gx0"] = gx0.reshape(-1, 3)
+        return out
+
 
 @dataclass
 class Replica:
