@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--path", default="auto", choices=["auto", "sparse", "dense"])
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 512-replica batched-ensemble measurement")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the differentiable AO fit (adjoint step time) measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 device-to-device copies of 512 MiB (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -215,6 +217,12 @@ def main():
         from proteinsim2015_amd import bench_ensemble
         ensemble = bench_ensemble.run(512, steps=500, baseline_steps=20)
         print(f"bench: ensemble speedup {ensemble['speedup']:.2f}x", file=sys.stderr, flush=True)
+    adjoint = None
+    if not args.no_adjoint:
+        from proteinsim2015_amd import bench_adjoint
+        adjoint = bench_adjoint.run()
+        print(f"bench: adjoint {adjoint['adjoint_ms_per_step']:.3f} ms per step",
+              file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline(pos, geom.box, args.cpu_steps)
     line = {
         "metric": "particle-steps/s at N=2M",
@@ -258,6 +266,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched_ensemble": ensemble,
+        "adjoint": adjoint,
     }
     print(json.dumps(line))
 

@@ -124,8 +124,12 @@ void record(particle **p, int n, double *pos, double *pos0, double *vel,
         pos[3 * i + 0] = p[i]->getX();  pos[3 * i + 1] = p[i]->getY();  pos[3 * i + 2] = p[i]->getZ();
         pos0[3 * i + 0] = p[i]->getX0(); pos0[3 * i + 1] = p[i]->getY0(); pos0[3 * i + 2] = p[i]->getZ0();
         vel[3 * i + 0] = p[i]->getVX(); vel[3 * i + 1] = p[i]->getVY(); vel[3 * i + 2] = p[i]->getVZ();
-        frc[3 * i + 0] = p[i]->getFX(); frc[3 * i + 1] = p[i]->getFY(); frc[3 * i + 2] = p[i]->getFZ();
-        frc0[3 * i + 0] = p[i]->getFX0(); frc0[3 * i + 1] = p[i]->getFY0(); frc0[3 * i + 2] = p[i]->getFZ0();
+        if (frc) {
+            frc[3 * i + 0] = p[i]->getFX(); frc[3 * i + 1] = p[i]->getFY(); frc[3 * i + 2] = p[i]->getFZ();
+        }
+        if (frc0) {
+            frc0[3 * i + 0] = p[i]->getFX0(); frc0[3 * i + 1] = p[i]->getFY0(); frc0[3 * i + 2] = p[i]->getFZ0();
+        }
     }
 }
 
@@ -229,6 +233,46 @@ int ref_run(const char *so_path, const char *cfg_path, int n,
         rec += omp_get_wtime() - r0;
     }
     if (loop_seconds) *loop_seconds = omp_get_wtime() - t0 - rec;
+    for (int i = 0; i < n; i++) delete p[i];
+    delete[] p;
+    delete integ;
+    return 0;
+}
+
+// As ref_run, recording positions, re-imaged previous positions and
+// velocities only every `every` steps (slot k = state after k*every steps;
+// nsteps/every + 1 slots), for long statistical runs.
+int ref_run_sampled(const char *so_path, const char *cfg_path, int n,
+                    const double *pos_in, const double *vel_in, double radius,
+                    double mass, int box, int cell, int scale, double dt, int nthreads,
+                    int nsteps, int every, double *tr_pos, double *tr_pos0,
+                    double *tr_vel)
+{
+    if (every <= 0) return -2;
+    config cfg(cfg_path);
+    IForce *f = load_force(so_path, &cfg);
+    if (!f) return -1;
+    brownianIntegrator *integ = new brownianIntegrator(&cfg);
+    particle **p = make_particles(n, pos_in, vel_in, radius, mass, box);
+    systemState st = make_state(n, box, cell, scale, dt, 0);
+    defaultForceManager *mgr = new defaultForceManager();
+    mgr->addForce(f);
+    mgr->setNumThreads(nthreads > 0 ? nthreads : 1);
+    std::vector<double> force(3 * (size_t)n, 0.0);
+    size_t stride = 3 * (size_t)n;
+    record(p, n, tr_pos, tr_pos0, tr_vel, nullptr, nullptr);
+    Cells c;
+    for (int s = 0; s < nsteps; s++) {
+        rebuild(p, st, c);
+        mgr->getAcceleration(c.sorted.data(), force.data(), &c.hash_index, &c.start_end, &st);
+        for (int i = 0; i < n; i++) p[i]->setForce(&force[3 * (size_t)i]);
+        integ->nextSystem(p, &st);
+        st.currentTime += st.dTime;
+        if ((s + 1) % every == 0) {
+            size_t o = (size_t)((s + 1) / every) * stride;
+            record(p, n, tr_pos + o, tr_pos0 + o, tr_vel + o, nullptr, nullptr);
+        }
+    }
     for (int i = 0; i < n; i++) delete p[i];
     delete[] p;
     delete integ;

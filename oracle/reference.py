@@ -52,6 +52,11 @@ class Reference:
                               C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                               C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
         L.ref_run.restype = C.c_int
+        L.ref_run_sampled.argtypes = [C.c_char_p, C.c_char_p, C.c_int, _dp, _dp, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
+                                      C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]
+        L.ref_run_sampled.restype = C.c_int
 
     @staticmethod
     def plugin(name):
@@ -108,3 +113,26 @@ class Reference:
         if not record:
             return secs.value
         return {k: v.reshape(nsteps + 1, n, 3) for k, v in tr.items()}
+
+    def run_sampled(self, plugin, cfg: dict, pos, vel, radius, mass, box, cell, scale, dt,
+                    nsteps, every, nthreads=8):
+        """Positions, re-imaged previous positions and velocities every
+        ``every`` steps: arrays [nsteps // every + 1, n, 3]."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        n = len(pos)
+        nf = nsteps // every + 1
+        tr = {k: np.empty(nf * 3 * n) for k in ("pos", "pos0", "vel")}
+        path = write_cfg(cfg)
+        try:
+            st = self.lib.ref_run_sampled(self.plugin(plugin), path.encode(), n, pos.ravel(),
+                                          vel.ravel(), float(radius), float(mass), int(box),
+                                          int(cell), int(scale), float(dt), int(nthreads),
+                                          int(nsteps), int(every),
+                                          *(tr[k].ctypes.data_as(C.c_void_p)
+                                            for k in ("pos", "pos0", "vel")))
+        finally:
+            os.unlink(path)
+        if st:
+            raise RuntimeError(f"ref_run_sampled failed ({st})")
+        return {k: v.reshape(nf, n, 3) for k, v in tr.items()}

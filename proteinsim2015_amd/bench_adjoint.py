@@ -1,0 +1,101 @@
+"""Differentiable AO fit workload: adjoint step time and gradient checks.
+
+N = 32768 colloids at volume fraction 0.15 (seeded RSA start), K = 200
+Brownian steps, dt = 1e-3, fixed in-kernel noise, checkpoints every 20
+steps.  Loss: squared distance between the Gaussian-smoothed g(r) of the
+last configuration (bins 0.8-3.0, width 0.02, sigma 0.02) and a target g(r)
+made by the same engine at U = 5 kT, q = 0.12; gradients with respect to
+(U, q) at U = 4 kT, q = 0.10 through ``fit.TrajectoryGofR``.
+
+    python -m proteinsim2015_amd.bench_adjoint [--validate]
+
+--validate adds central differences of the loss at relative steps 1e-6 and
+1e-4.  The reference's AO force jumps at the cutoff (36 c^-38); at this size
+a step of 1e-4 in U or q, and any feasible step in q, moves pairs across the
+cutoff during the trajectory, and the secant then includes those jumps,
+which the pathwise derivative excludes.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import fit
+
+
+def _loss(prob, u, q, target):
+    g = prob.forward(u, q)
+    return float(np.sum((g - target) ** 2)), g
+
+
+def run(n: int = 32768, nsteps: int = 200, every: int = 20, reps: int = 3,
+        validate: bool = False, verbose: bool = False) -> dict:
+    t0 = time.perf_counter()
+    prob = fit.c5_problem(n=n, nsteps=nsteps, every=every)
+    target = prob.forward(5.0, 0.12)
+    if verbose:
+        print(f"adjoint bench: setup + target {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+              flush=True)
+    tgt = torch.from_numpy(target)
+    theta = torch.tensor([4.0, 0.10], dtype=torch.float64, requires_grad=True)
+    fit.gofr_loss(prob, theta, tgt).backward()      # warm-up
+    tf = tb = 0.0
+    for _ in range(reps):
+        theta.grad = None
+        a = time.perf_counter()
+        loss = fit.gofr_loss(prob, theta, tgt)
+        b = time.perf_counter()
+        loss_value = float(loss.detach())
+        loss.backward()
+        c = time.perf_counter()
+        tf += b - a
+        tb += c - b
+    grad = theta.grad.tolist()
+    out = {
+        "workload": f"AO fit N={n} phi=0.15 K={nsteps} checkpoint every {every}",
+        "n": n, "nsteps": nsteps, "checkpoint_every": every,
+        "loss": loss_value, "grad_U": grad[0], "grad_q": grad[1],
+        "forward_ms": tf / reps * 1e3,
+        "backward_ms": tb / reps * 1e3,
+        "forward_ms_per_step": tf / reps / nsteps * 1e3,
+        "adjoint_ms_per_step": tb / reps / nsteps * 1e3,
+    }
+    if validate:
+        out.update(_validate(prob, target, grad, verbose))
+    prob.close()
+    prob.system.close()
+    return out
+
+
+def _validate(prob, target, grad, verbose):
+    res = {}
+    u0, q0 = 4.0, 0.10
+    for name, idx, h in (("U", 0, 1e-6 * u0), ("U_step1e-4", 0, 1e-4 * u0),
+                         ("q", 1, 1e-6 * q0), ("q_step1e-4", 1, 1e-4 * q0)):
+        lp, _ = _loss(prob, u0 + (h if idx == 0 else 0), q0 + (h if idx == 1 else 0), target)
+        lm, _ = _loss(prob, u0 - (h if idx == 0 else 0), q0 - (h if idx == 1 else 0), target)
+        fd = (lp - lm) / (2 * h)
+        res[f"fd_{name}"] = fd
+        res[f"fd_{name}_rel_err"] = abs(grad[idx] - fd) / abs(fd)
+        if verbose:
+            print(f"fd {name}: {fd:.9g} adjoint {grad[idx]:.9g}", file=sys.stderr, flush=True)
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--every", type=int, default=20)
+    ap.add_argument("--validate", action="store_true")
+    args = ap.parse_args()
+    print(json.dumps(run(args.n, args.steps, args.every, validate=args.validate, verbose=True)))
+
+
+if __name__ == "__main__":
+    main()

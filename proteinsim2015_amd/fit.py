@@ -133,19 +133,22 @@ def c5_problem(n: int = 32768, phi: float = 0.15, nsteps: int = 200, every: int 
 
 
 def fit(problem: AOFitProblem, theta0, target_g: np.ndarray, iters: int = 20,
-        lr: float = 0.05):
-    """Adam on (U, q) against a target g(r); returns the history."""
-    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+        lr: float = 0.05, scale=(1.0, 0.1)):
+    """Adam on (U, q) against a target g(r), in coordinates (U, q) / scale so
+    that one learning rate suits both; returns [(theta, loss, dL/dtheta)]."""
+    sc = torch.tensor(scale, dtype=torch.float64)
+    z = (torch.tensor(theta0, dtype=torch.float64) / sc).requires_grad_(True)
     target = torch.from_numpy(np.asarray(target_g, dtype=np.float64))
-    opt = torch.optim.Adam([theta], lr=lr)
+    opt = torch.optim.Adam([z], lr=lr)
     hist = []
     for _ in range(iters):
         opt.zero_grad()
+        theta = z * sc
         loss = gofr_loss(problem, theta, target)
         loss.backward()
-        hist.append((theta.detach().tolist(), float(loss), theta.grad.tolist()))
+        hist.append((theta.detach().tolist(), float(loss), (z.grad / sc).tolist()))
         opt.step()
         with torch.no_grad():
-            theta[1].clamp_(0.02, 0.2)
-            theta[0].clamp_(0.5, 12.0)
+            z[0].clamp_(0.5 / scale[0], 12.0 / scale[0])
+            z[1].clamp_(0.02 / scale[1], 0.2 / scale[1])
     return hist
