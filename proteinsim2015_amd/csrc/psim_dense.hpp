@@ -20,19 +20,32 @@
 
 namespace psim {
 
-constexpr int kDenseThreads = 128;   // two wave64s per home cell
-constexpr int kStageCap = 512;       // staged candidates per chunk
-constexpr int kDenseHitCap = 32;     // per-lane hit-list entries
+// Tile shape (overridable at build time for launch-shape sweeps).
+#ifndef PSIM_DENSE_THREADS
+#define PSIM_DENSE_THREADS 256
+#endif
+#ifndef PSIM_STAGE_CAP
+#define PSIM_STAGE_CAP 512
+#endif
+#ifndef PSIM_DENSE_HIT_CAP
+#define PSIM_DENSE_HIT_CAP 16
+#endif
+constexpr int kDenseThreads = PSIM_DENSE_THREADS;  // lanes per home cell (4 wave64s)
+constexpr int kStageCap = PSIM_STAGE_CAP;          // staged candidates per chunk
+constexpr int kDenseHitCap = PSIM_DENSE_HIT_CAP;   // per-lane hit-list entries
+static_assert(kDenseThreads >= 64 && kDenseThreads % 64 == 0, "whole wave64s per cell");
 
-struct DenseShared {
+template <int T, int HC>
+struct DenseSharedT {
     double sx[kStageCap], sy[kStageCap], sz[kStageCap];
     int sslot[kStageCap];
-    unsigned short hits[kDenseHitCap * kDenseThreads];
-    double red[3][kDenseThreads];
+    unsigned short hits[HC * T];
+    double red[3][T];
     int coff[28];
     int cbase[27];
     double shx[27], shy[27], shz[27];
 };
+using DenseShared = DenseSharedT<kDenseThreads, kDenseHitCap>;
 
 // Map a workgroup index to a cell so that each XCD (workgroups are handed
 // out round-robin over 8 XCDs) walks one contiguous slab of cells and its
@@ -42,8 +55,8 @@ __device__ inline int xcd_cell(int b, int per_xcd)
     return (b & 7) * per_xcd + (b >> 3);
 }
 
-template <int MASK>
-__device__ void dense_drain(const Geo &g, const Pot &p, const DenseShared &sh,
+template <int MASK, int T, int HC>
+__device__ void dense_drain(const Geo &g, const Pot &p, const DenseSharedT<T, HC> &sh,
                             const double *__restrict__ pos, double xi, double yi,
                             double zi, double size, double ov2_hi, int self, int lane,
                             int nh, double &fx,
@@ -52,7 +65,7 @@ __device__ void dense_drain(const Geo &g, const Pot &p, const DenseShared &sh,
 {
     const int N = g.n;
     for (int k = 0; k < nh; ++k) {
-        const int j = sh.hits[k * kDenseThreads + lane];
+        const int j = sh.hits[k * T + lane];
         if (j == self) continue;
         const double dx = sh.sx[j] - xi, dy = sh.sy[j] - yi, dz = sh.sz[j] - zi;
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
@@ -78,12 +91,12 @@ __device__ void dense_drain(const Geo &g, const Pot &p, const DenseShared &sh,
 // Forces on the home particles of cell c.  On return, lanes 0..ni-1 hold the
 // total force on home particle (i0 + lane) for each i-chunk via the callback
 // epi(i, fx, fy, fz) (called by those lanes only).
-template <int MASK, class Epi>
+template <int MASK, int T, int HC, class Epi>
 __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
                                   const double *__restrict__ pos,
                                   const int *__restrict__ cstart,
                                   const int *__restrict__ id, DevErr *err, int c,
-                                  DenseShared &sh, Epi epi)
+                                  DenseSharedT<T, HC> &sh, Epi epi)
 {
     const int N = g.n, S = g.S;
     const int lane = threadIdx.x;
@@ -126,9 +139,9 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
     const double ov = p.ov_factor * size;
     const double ov2_hi = ov * ov * (1.0 + p.ov_band);
 
-    for (int i0 = 0; i0 < ni; i0 += kDenseThreads) {
-        const int nic = min(kDenseThreads, ni - i0);
-        const int G = kDenseThreads / nic;
+    for (int i0 = 0; i0 < ni; i0 += T) {
+        const int nic = min(T, ni - i0);
+        const int G = T / nic;
         const int gi = lane % nic, gg = lane / nic;
         const bool act = gg < G;
         const int si = home0 + i0 + gi;       // global slot of this lane's i
@@ -139,18 +152,21 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
         for (int s0 = 0; s0 < M; s0 += kStageCap) {
             const int mc = min(kStageCap, M - s0);
             __syncthreads();   // the previous chunk is no longer read
-            for (int k = lane; k < mc; k += kDenseThreads) {
-                const int t = s0 + k;
-                int lo = 0, hi2 = 26;   // last q with coff[q] <= t
-                while (lo < hi2) {
-                    const int mid = (lo + hi2 + 1) >> 1;
-                    if (sh.coff[mid] <= t) lo = mid; else hi2 = mid - 1;
+            // kPer lanes per neighbour cell copy its particles that fall in
+            // this chunk (no search for the cell of a staged index)
+            constexpr int kPer = T / 27;
+            if (lane < 27 * kPer) {
+                const int q = lane / kPer, u = lane % kPer;
+                const int lo = max(sh.coff[q], s0), hi = min(sh.coff[q + 1], s0 + mc);
+                const int base = sh.cbase[q] - sh.coff[q];
+                const double ex = sh.shx[q], ey = sh.shy[q], ez = sh.shz[q];
+                for (int t = lo + u; t < hi; t += kPer) {
+                    const int slot = base + t, k = t - s0;
+                    sh.sx[k] = pos[slot] + ex;
+                    sh.sy[k] = pos[N + slot] + ey;
+                    sh.sz[k] = pos[2 * N + slot] + ez;
+                    sh.sslot[k] = slot;
                 }
-                const int slot = sh.cbase[lo] + (t - sh.coff[lo]);
-                sh.sx[k] = pos[slot] + sh.shx[lo];
-                sh.sy[k] = pos[N + slot] + sh.shy[lo];
-                sh.sz[k] = pos[2 * N + slot] + sh.shz[lo];
-                sh.sslot[k] = slot;
             }
             __syncthreads();
             if (act) {
@@ -163,11 +179,11 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
                         const int jj = j + u * G;
                         const double dx = sh.sx[jj] - xi, dy = sh.sy[jj] - yi, dz = sh.sz[jj] - zi;
                         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                        sh.hits[nh * kDenseThreads + lane] = (unsigned short)jj;
+                        sh.hits[nh * T + lane] = (unsigned short)jj;
                         nh += (r2 < p.rc2_hi) ? 1 : 0;
                     }
-                    if (__any(nh > kDenseHitCap - 4)) {
-                        dense_drain<MASK>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, hi - s0,
+                    if (__any(nh > HC - 4)) {
+                        dense_drain<MASK, T, HC>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, hi - s0,
                                           lane, nh, fx, fy, fz, err, id_i, id);
                         nh = 0;
                     }
@@ -175,10 +191,10 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
                 for (; j < mc; j += G) {
                     const double dx = sh.sx[j] - xi, dy = sh.sy[j] - yi, dz = sh.sz[j] - zi;
                     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                    sh.hits[nh * kDenseThreads + lane] = (unsigned short)j;
+                    sh.hits[nh * T + lane] = (unsigned short)j;
                     nh += (r2 < p.rc2_hi) ? 1 : 0;
                 }
-                dense_drain<MASK>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, hi - s0, lane, nh,
+                dense_drain<MASK, T, HC>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, hi - s0, lane, nh,
                                   fx, fy, fz, err, id_i, id);
             }
         }

@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_step_dense(Geo g, Pot p, doub
     if (c >= g.ncell) return;
     const int home0 = cstart[c];
     const int N = g.n;
-    dense_cell_forces<MASK>(g, p, radius, pos, cstart, id, err, c, sh,
+    dense_cell_forces<MASK, kDenseThreads, kDenseHitCap>(g, p, radius, pos, cstart, id, err, c, sh,
                             [=](int i, double Fx, double Fy, double Fz) {
                                 const int s = home0 + i;
                                 nfrc[s] = Fx;
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_forces_dense(Geo g, Pot p, do
     if (c >= g.ncell) return;
     const int home0 = cstart[c];
     const int N = g.n;
-    dense_cell_forces<MASK>(g, p, radius, pos, cstart, id, err, c, sh,
+    dense_cell_forces<MASK, kDenseThreads, kDenseHitCap>(g, p, radius, pos, cstart, id, err, c, sh,
                             [=](int i, double Fx, double Fy, double Fz) {
                                 const int o = id[home0 + i];
                                 fout[o] = Fx;
