@@ -41,6 +41,7 @@ using eng::dalloc;
 namespace {
 
 constexpr int kAdjBlock = 256;
+constexpr int kAdjSmall = 256 * 1024;   // below this N, 64-lane blocks fill the CUs
 constexpr int kNTheta = 5;   // AO wellDepth, AO cutOff, LJ wellDepth, yukawaStrength, debyeLength
 
 // Host-side derivatives of the pair constants (AOPotential.cpp:50-55).
@@ -167,13 +168,13 @@ __device__ void visit_pairs(const Geo &g, const Pot &p, const double *pos, const
 }
 
 // One backward step at the recorded state x_n.
-template <int MASK, bool EXACT>
-__global__ __launch_bounds__(kAdjBlock) void k_adj_step(
+template <int MASK, bool EXACT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_adj_step(
     Geo g, Pot p, AdjPot q, double radius, const double *pos, const int *id,
     const int *cstart, const double *lam, const double *a1, const double *a2, double *a0,
     double *lam_prev, double beta, double coeff0, double alpha_prev, double c2, double *tpart)
 {
-    const int s = blockIdx.x * kAdjBlock + threadIdx.x;
+    const int s = blockIdx.x * BLOCK + threadIdx.x;
     const int N = g.n;
     double th[kNTheta] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (s < N) {
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_step(
             lam_prev[2 * N + o] = alpha_prev * n0z - c2 * a1[2 * N + o];
         }
     }
-    __shared__ double red[kNTheta][kAdjBlock / 64];
+    __shared__ double red[kNTheta][BLOCK / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < kNTheta; ++k) {
@@ -222,19 +223,28 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_step(
     __syncthreads();
     if (threadIdx.x < kNTheta) {
         double t = 0.0;
-        for (int k = 0; k < kAdjBlock / 64; ++k) t += red[threadIdx.x][k];
+        for (int k = 0; k < BLOCK / 64; ++k) t += red[threadIdx.x][k];
         tpart[(size_t)blockIdx.x * kNTheta + threadIdx.x] = t;
     }
 }
 
-// acc[k] += sum over blocks of tpart[b][k], in block order (deterministic)
-__global__ void k_theta_accum(const double *tpart, int nb, double *acc)
+// acc[k] = sum of tpart[b][k] over all nb block partials of all steps:
+// block k, lane t sums b = t, t + 256, ... in order, then a fixed tree
+// (deterministic)
+__global__ __launch_bounds__(256) void k_theta_sum(const double *tpart, long long nb,
+                                                   double *acc)
 {
-    const int k = threadIdx.x;
-    if (k >= kNTheta) return;
-    double t = 0.0;
-    for (int b = 0; b < nb; ++b) t += tpart[(size_t)b * kNTheta + k];
-    acc[k] += t;
+    __shared__ double red[256];
+    const int k = blockIdx.x, t = threadIdx.x;
+    double v = 0.0;
+    for (long long b = t; b < nb; b += 256) v += tpart[(size_t)b * kNTheta + k];
+    red[t] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) acc[k] = red[0];
 }
 
 // lam = alpha * a1 (the last step: a_{K+1} = 0)
@@ -324,22 +334,27 @@ bool same_forces(const psim_params &a, const psim_params &b)
            a.debye_length == b.debye_length && a.ao_well_depth == b.ao_well_depth;
 }
 
+int adj_block(const psim_ctx *c) { return c->n < kAdjSmall ? 64 : kAdjBlock; }
+
+// one backward step; its block partials of dL/dtheta go to tpart at tp
 template <int MASK>
 void launch_adj_mask(psim_adjoint *A, const AdjPot &q, const double *pos, const int *id,
                      const int *cs, const double *lam, const double *a1, const double *a2,
-                     double *a0, double *lam_prev, double beta, double alpha_prev)
+                     double *a0, double *lam_prev, double beta, double alpha_prev, double *tp)
 {
     psim_ctx *c = A->c;
-    const unsigned nb = blocks_for(c->n, kAdjBlock);
-    if (c->geo.exact_mi)
-        k_adj_step<MASK, true><<<nb, kAdjBlock, 0, c->stream>>>(
-            c->geo, c->pot, q, c->prm.radius, pos, id, cs, lam, a1, a2, a0, lam_prev, beta,
-            c->gb.coeff0, alpha_prev, c->gb.c2, A->tpart);
-    else
-        k_adj_step<MASK, false><<<nb, kAdjBlock, 0, c->stream>>>(
-            c->geo, c->pot, q, c->prm.radius, pos, id, cs, lam, a1, a2, a0, lam_prev, beta,
-            c->gb.coeff0, alpha_prev, c->gb.c2, A->tpart);
-    k_theta_accum<<<1, 64, 0, c->stream>>>(A->tpart, (int)nb, A->acc);
+    const int B = adj_block(c);
+    const unsigned nb = blocks_for(c->n, B);
+#define PSIM_ADJ_LAUNCH(EX, BL)                                                            \
+    k_adj_step<MASK, EX, BL><<<nb, BL, 0, c->stream>>>(                                    \
+        c->geo, c->pot, q, c->prm.radius, pos, id, cs, lam, a1, a2, a0, lam_prev, beta,    \
+        c->gb.coeff0, alpha_prev, c->gb.c2, tp)
+    if (c->geo.exact_mi) {
+        if (B == 64) PSIM_ADJ_LAUNCH(true, 64); else PSIM_ADJ_LAUNCH(true, kAdjBlock);
+    } else {
+        if (B == 64) PSIM_ADJ_LAUNCH(false, 64); else PSIM_ADJ_LAUNCH(false, kAdjBlock);
+    }
+#undef PSIM_ADJ_LAUNCH
 }
 
 }  // namespace
@@ -359,7 +374,8 @@ int psim_adj_create(psim_ctx *c, int nsteps, int checkpoint_every, psim_adjoint 
     A->meta.resize(A->ncp + 1);
     A->first.assign(nsteps, 0);
     const size_t n3 = 3 * (size_t)c->n;
-    const unsigned nb = blocks_for(c->n, kAdjBlock);
+    // block partials of dL/dtheta for every step, summed once at the end
+    const size_t nb = blocks_for(c->n, adj_block(c));
     int rc = 0;
     if (!rc) rc = dalloc(&A->cp_d, (size_t)(A->ncp + 1) * cp_stride_d(c));
     if (!rc) rc = dalloc(&A->cp_i, (size_t)(A->ncp + 1) * cp_stride_i(c));
@@ -367,7 +383,7 @@ int psim_adj_create(psim_ctx *c, int nsteps, int checkpoint_every, psim_adjoint 
     if (!rc) rc = dalloc(&A->rec_i, (size_t)A->every * cp_stride_i(c));
     for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(&A->a[k], n3);
     for (int k = 0; k < 2 && !rc; ++k) rc = dalloc(&A->lam[k], n3);
-    if (!rc) rc = dalloc(&A->tpart, (size_t)nb * kNTheta);
+    if (!rc) rc = dalloc(&A->tpart, (size_t)nsteps * nb * kNTheta);
     if (!rc) rc = dalloc(&A->acc, kNTheta);
     if (!rc) rc = save_cp(A, 0);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = PSIM_E_HIP;
@@ -450,7 +466,7 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
         CK(hipStreamSynchronize(c->stream));
     }
     CK(hipMemsetAsync(A->a[2], 0, sizeof(double) * n3, c->stream));
-    CK(hipMemsetAsync(A->acc, 0, sizeof(double) * kNTheta, c->stream));
+    const size_t nbk = blocks_for(n, adj_block(c));
     k_scale<<<blocks_for((long long)n3, 256), 256, 0, c->stream>>>((int)n3, alpha(A->K - 1),
                                                                    A->a[1], A->lam[0]);
     CK(hipGetLastError());
@@ -480,14 +496,15 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
             const int *cs = id + n;
             double *lp = s >= 1 ? lam_prev : nullptr;
             const double ap = s >= 1 ? alpha(s - 1) : 0.0;
+            double *tp = A->tpart + (size_t)s * nbk * kNTheta;
             switch (c->pot.mask) {
-            case 1: launch_adj_mask<1>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
-            case 2: launch_adj_mask<2>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
-            case 3: launch_adj_mask<3>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
-            case 4: launch_adj_mask<4>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
-            case 5: launch_adj_mask<5>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
-            case 6: launch_adj_mask<6>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
-            case 7: launch_adj_mask<7>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap); break;
+            case 1: launch_adj_mask<1>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
+            case 2: launch_adj_mask<2>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
+            case 3: launch_adj_mask<3>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
+            case 4: launch_adj_mask<4>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
+            case 5: launch_adj_mask<5>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
+            case 6: launch_adj_mask<6>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
+            case 7: launch_adj_mask<7>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
             default: return PSIM_E_ARG;
             }
             CK(hipGetLastError());
@@ -501,6 +518,8 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
             lam_prev = t;
         }
     }
+    k_theta_sum<<<kNTheta, 256, 0, c->stream>>>(A->tpart, (long long)A->K * nbk, A->acc);
+    CK(hipGetLastError());
     // a1 now holds a_0 = dL/dx_0
     std::vector<double> h(gx0 ? n3 : 0);
     if (gx0)

@@ -101,9 +101,13 @@ void launch_step_mask(psim_ctx *c, const StepArgs &a)
         const int per = (c->ncell + 7) / 8;
         k_step_dense<MASK><<<8 * per, kDenseThreads, 0, c->stream>>>(
             c->geo, c->pot, c->prm.radius, a.pos, a.id, a.cstart, a.nfrc, a.err, per);
+    } else if (c->n < kSmallSystem) {
+        // few particles: small blocks so that every CU gets work
+        k_step<MASK, 64><<<blocks_for(c->n, 64), 64, 0, c->stream>>>(c->geo, c->pot, c->gb,
+                                                                     c->prm.radius, a);
     } else {
-        k_step<MASK><<<blocks_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->geo, c->pot, c->gb,
-                                                                        c->prm.radius, a);
+        k_step<MASK, kBlock><<<blocks_for(c->n, kBlock), kBlock, 0, c->stream>>>(
+            c->geo, c->pot, c->gb, c->prm.radius, a);
     }
 }
 

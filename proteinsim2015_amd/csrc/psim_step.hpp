@@ -10,6 +10,9 @@ namespace {
 
 // mean particles per cell above which the cell-per-workgroup kernel is used
 constexpr double kDenseMinOccupancy = 2.0;
+// below this many particles the thread-per-particle kernels use 64-lane
+// blocks (256-lane blocks would leave CUs idle: 256 CUs x 4 blocks)
+constexpr int kSmallSystem = 256 * 1024;
 
 // Step control kept in device memory so that a captured sequence of steps
 // can be replayed: the step kernels read it, the reorder kernel advances it.
@@ -148,12 +151,12 @@ __device__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int
 }
 
 // Thread-per-particle step (sparse systems and S < 4).
-template <int MASK>
-__global__ __launch_bounds__(kBlock) void k_step(Geo g, Pot p, GB gb, double radius,
-                                                 StepArgs a)
+template <int MASK, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_step(Geo g, Pot p, GB gb, double radius,
+                                                StepArgs a)
 {
-    __shared__ unsigned hits[kHitCap * kBlock];
-    const int s = blockIdx.x * kBlock + threadIdx.x;
+    __shared__ unsigned hits[kHitCap * BLOCK];
+    const int s = blockIdx.x * BLOCK + threadIdx.x;
     if (s >= g.n) return;
     const int N = g.n;
     const double xi = a.pos[s], yi = a.pos[N + s], zi = a.pos[2 * N + s];
@@ -162,11 +165,11 @@ __global__ __launch_bounds__(kBlock) void k_step(Geo g, Pot p, GB gb, double rad
     StartTab tab{a.cstart};
     Orig orig{a.id, 1, 0};
     if (g.exact_mi)
-        gather_force<MASK, true>(g, p, src, tab, s, xi, yi, zi, radius,
-                                 hits + threadIdx.x, fx, fy, fz, a.err, orig);
+        gather_force<MASK, true, BLOCK>(g, p, src, tab, s, xi, yi, zi, radius,
+                                        hits + threadIdx.x, fx, fy, fz, a.err, orig);
     else
-        gather_force<MASK, false>(g, p, src, tab, s, xi, yi, zi, radius,
-                                  hits + threadIdx.x, fx, fy, fz, a.err, orig);
+        gather_force<MASK, false, BLOCK>(g, p, src, tab, s, xi, yi, zi, radius,
+                                         hits + threadIdx.x, fx, fy, fz, a.err, orig);
     integrate_one<true>(g, gb, a, s, fx, fy, fz);
 }
 
