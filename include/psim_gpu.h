@@ -36,6 +36,9 @@
  *   psim_gpu_last_error    error::throw* (error.cpp:40-62), as a code
  *   psim_init_rsa          system::initParticles + initCheck
  *                          (systemInit.cpp:31-160), O(N) with a cell grid
+ *   psim_init_reference    system::initParticles + initCheck +
+ *                          maxwellVelocityInit (systemInit.cpp:31-267),
+ *                          the same draws in the same order
  *   psim_gpu_set_force_params  the plugin constructors' parameter reads
  *                          (LJPotential.cpp:34-55, AOPotential.cpp:40-55),
  *                          applied to an existing context
@@ -297,6 +300,18 @@ int psim_adj_forward(psim_adjoint *adj);
  * force parameters must be those of the last psim_adj_forward; the context
  * is left at step nsteps. */
 int psim_adj_backward(psim_adjoint *adj, const double *gx, double *dtheta, double *gx0);
+
+/* The reference's own initial configuration: n particles of radius r in a
+ * periodic cube of integer edge box, std::mt19937(seed) with
+ * std::uniform_real_distribution<double>(0, 1) draws in the reference's
+ * order (seed cube of seed_size^3 particles at the centre, uniform
+ * placement, overlap re-draws while the squared distance is below 2r, then
+ * Maxwell velocities at the temperature truncated to an integer).  pos,
+ * pos0 (the setters' previous-position image) and vel are AoS3; pos0 and
+ * vel may be NULL.  Returns 0, PSIM_ERR_INIT (more than 10 n re-draws) or
+ * PSIM_E_ARG.  Host only. */
+int psim_init_reference(int n, int box, double radius, unsigned int seed, int seed_size,
+                        double temp, double *pos, double *pos0, double *vel);
 
 /* Seeded random sequential addition of n spheres with pair distance >= dmin
  * in a periodic cube [0, box)^3 (host, O(n)).  Returns 0 or PSIM_ERR_INIT. */

@@ -20,6 +20,8 @@
 #include "integrator.h"
 #include "utilities.h"
 #include "fastRandom.h"
+#include "RecoverySystem.h"
+#include "system.h"
 
 #include <dlfcn.h>
 #include <parallel/algorithm>
@@ -276,6 +278,29 @@ int ref_run_sampled(const char *so_path, const char *cfg_path, int n,
     for (int i = 0; i < n; i++) delete p[i];
     delete[] p;
     delete integ;
+    return 0;
+}
+
+// The reference's own system construction (system.cpp:108-140): the
+// constructor places the particles (systemInit.cpp:31-267) and writes
+// <trial_dir>/sysConfig.cfg, <trial_dir>/initialState.txt (recovery format,
+// analysisOutput.cpp:53-74) and <trial_dir>/movie/system-initial.xyz.  The
+// directories must exist.  The system is not destroyed (its destructor
+// deletes the integrator and force manager it was handed, here none).
+int ref_init(const char *cfg_path, const char *trial_dir)
+{
+    config cfg(cfg_path);
+    new PSim::system(&cfg, nullptr, nullptr, trial_dir);
+    return 0;
+}
+
+// RecoverySystem (RecoverySystem.cpp:27-43): reads
+// <sys_state>/snapshots/time-<stamp>/recovery.txt (systemRecovery.cpp:43-81)
+// and writes the loaded particles to <sys_state>/-rewind-<stamp>/initialState.txt.
+int ref_recover(const char *cfg_path, const char *sys_state, const char *stamp)
+{
+    config cfg(cfg_path);
+    new PSim::RecoverySystem(&cfg, sys_state, stamp, nullptr, nullptr);
     return 0;
 }
 

@@ -57,6 +57,10 @@ class Reference:
                                       C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                       C.c_void_p]
         L.ref_run_sampled.restype = C.c_int
+        L.ref_init.argtypes = [C.c_char_p, C.c_char_p]
+        L.ref_init.restype = C.c_int
+        L.ref_recover.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
+        L.ref_recover.restype = C.c_int
 
     @staticmethod
     def plugin(name):
@@ -136,3 +140,31 @@ class Reference:
         if st:
             raise RuntimeError(f"ref_run_sampled failed ({st})")
         return {k: v.reshape(nf, n, 3) for k, v in tr.items()}
+
+    def init(self, cfg: dict, trial_dir: str) -> None:
+        """Construct the reference's system in trial_dir (which receives
+        sysConfig.cfg, initialState.txt and movie/system-initial.xyz)."""
+        os.makedirs(os.path.join(trial_dir, "movie"), exist_ok=True)
+        path = write_cfg(cfg)
+        try:
+            st = self.lib.ref_init(path.encode(), trial_dir.encode())
+        finally:
+            os.unlink(path)
+        if st:
+            raise RuntimeError(f"ref_init failed ({st})")
+
+
+    def recover(self, cfg: dict, sys_state: str, stamp: str) -> str:
+        """Load sys_state/snapshots/time-<stamp>/recovery.txt with the
+        reference's RecoverySystem; returns the directory holding the
+        initialState.txt it writes for the loaded particles."""
+        rewind = os.path.join(sys_state, f"-rewind-{stamp}")
+        os.makedirs(os.path.join(rewind, "movie"), exist_ok=True)
+        path = write_cfg(cfg)
+        try:
+            st = self.lib.ref_recover(path.encode(), sys_state.encode(), stamp.encode())
+        finally:
+            os.unlink(path)
+        if st:
+            raise RuntimeError(f"ref_recover failed ({st})")
+        return rewind

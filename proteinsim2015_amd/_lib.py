@@ -69,7 +69,7 @@ EXPORTS = (
     "psim_gpu_set_force_params", "psim_gpu_track_start", "psim_gpu_observables",
     "psim_gpu_clusters", "psim_gpu_pair_histogram", "psim_gpu_pair_histogram_grad",
     "psim_adj_create", "psim_adj_destroy", "psim_adj_set_initial", "psim_adj_forward",
-    "psim_adj_backward",
+    "psim_adj_backward", "psim_init_reference",
 )
 
 _vp = C.c_void_p
@@ -121,6 +121,8 @@ _SIGS = {
     "psim_adj_set_initial": ([_vp], C.c_int),
     "psim_adj_forward": ([_vp], C.c_int),
     "psim_adj_backward": ([_vp, _dp, _dp, _dp], C.c_int),
+    "psim_init_reference": ([C.c_int, C.c_int, C.c_double, C.c_uint, C.c_int, C.c_double, _dp,
+                             _dp, _dp], C.c_int),
 }
 
 _LIB = None

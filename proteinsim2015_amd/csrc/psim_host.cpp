@@ -166,3 +166,209 @@ extern "C" int psim_init_rsa(int n, double box, double dmin, unsigned long long 
     }
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// The reference's own initial configuration
+// ---------------------------------------------------------------------------
+// psim_init_reference replays system::initParticles, initCheck and
+// maxwellVelocityInit (systemInit.cpp:31-267): the same generator
+// (std::mt19937 seeded with the config seed), the same
+// std::uniform_real_distribution<double>(0, 1) draws in the same order, the
+// particle setters' single wrap and previous-position image
+// (particle.cpp:61-99, utilities.cpp:26-50), and the same libm calls, so with
+// the same C++ library it reproduces the reference's positions and velocities
+// bit for bit.  initCheck's O(N^2) scan (each particle against every other
+// one in index order, re-drawn on the first overlap found and the scan
+// continued with the new position) is replayed with a cell grid that yields
+// the overlapping particles in index order, which keeps the draw sequence.
+#include <algorithm>
+#include <random>
+
+namespace {
+
+struct RefParticles {
+    int n = 0;
+    double box = 0;
+    std::vector<double> x, x0;   // AoS3
+
+    // particle::setX/Y/Z (particle.cpp:61-92)
+    void set(int i, int d, double val)
+    {
+        const double old = x[3 * (size_t)i + d];
+        double v = val;
+        if (v < 0) v = v + box;            // util::safeMod
+        else if (v >= box) v = v - box;
+        x[3 * (size_t)i + d] = v;
+        const double dx = v - old;          // util::safeMod0
+        x0[3 * (size_t)i + d] = (std::fabs(dx) > (box / 2)) ? ((dx < 0) ? old - box : old + box)
+                                                             : old;
+    }
+};
+
+// util::pbcDist (utilities.cpp:79-104)
+inline double ref_pbc_dist(const double *a, const double *b, double L)
+{
+    double dx = std::fabs(a[0] - b[0]), dy = std::fabs(a[1] - b[1]), dz = std::fabs(a[2] - b[2]);
+    const double half = L / 2.0;
+    if (dx > half) dx -= L;
+    if (dy > half) dy -= L;
+    if (dz > half) dz -= L;
+    return (dx * dx) + (dy * dy) + (dz * dz);
+}
+
+struct IndexGrid {
+    int G = 0;
+    double edge = 0;
+    std::vector<std::vector<int>> cells;
+    std::vector<int> where;
+
+    int c1(double v) const
+    {
+        int c = (int)(v / edge);
+        return c < 0 ? 0 : (c >= G ? G - 1 : c);
+    }
+    int cell(const double *p) const { return (c1(p[2]) * G + c1(p[1])) * G + c1(p[0]); }
+    void insert(int i, const double *p)
+    {
+        const int c = cell(p);
+        cells[c].push_back(i);
+        where[i] = c;
+    }
+    void remove(int i)
+    {
+        auto &v = cells[where[i]];
+        v.erase(std::find(v.begin(), v.end(), i));
+    }
+};
+
+}  // namespace
+
+extern "C" int psim_init_reference(int n, int box, double radius, unsigned int seed,
+                                   int seed_size, double temp, double *pos_out,
+                                   double *pos0_out, double *vel_out)
+{
+    if (n <= 0 || box <= 0 || !(radius > 0) || seed_size < 0 || !pos_out) return PSIM_E_ARG;
+    const int seed_cube = seed_size * seed_size * seed_size;
+    if (seed_cube > n) return PSIM_E_ARG;
+    RefParticles P;
+    P.n = n;
+    P.box = box;
+    P.x.assign(3 * (size_t)n, 0.0);
+    P.x0.assign(3 * (size_t)n, 0.0);
+    std::mt19937 gen(seed);
+    std::uniform_real_distribution<double> distribution(0.0, 1.0);
+    // seed cube (systemInit.cpp:45-76)
+    if (seed_size > 0) {
+        int lower, upper;
+        if (seed_size % 2 == 0) {
+            lower = -seed_size / 2;
+            upper = seed_size / 2;
+        } else {
+            lower = -seed_size / 2;
+            upper = (seed_size / 2) + 1;
+        }
+        int k = 0;
+        for (int x = lower; x < upper; x++)
+            for (int y = lower; y < upper; y++)
+                for (int z = lower; z < upper; z++) {
+                    float center = box / 2.0;
+                    P.set(k, 0, center + (2.1 * x * radius));
+                    P.set(k, 1, center + (2.1 * y * radius));
+                    P.set(k, 2, center + (2.1 * z * radius));
+                    k++;
+                }
+    }
+    for (int i = seed_cube; i < n; i++) {   // :80-89
+        P.set(i, 0, distribution(gen) * box);
+        P.set(i, 1, distribution(gen) * box);
+        P.set(i, 2, distribution(gen) * box);
+    }
+    // initCheck (:99-160): overlap when pbcDist (a squared distance) is below
+    // the sum of the radii, i.e. distance < sqrt(2 r)
+    const double rsum = radius + radius;
+    const double reach = std::sqrt(rsum);
+    IndexGrid grid;
+    grid.G = (int)std::floor(box / reach);
+    const bool use_grid = grid.G >= 3;
+    if (use_grid) {
+        grid.edge = (double)box / grid.G;
+        grid.cells.assign((size_t)grid.G * grid.G * grid.G, {});
+        grid.where.assign(n, 0);
+        for (int i = 0; i < n; i++) grid.insert(i, &P.x[3 * (size_t)i]);
+    }
+    // smallest j >= from (j != i) overlapping particle i at its current position
+    auto next_overlap = [&](int i, int from) -> int {
+        const double *pi = &P.x[3 * (size_t)i];
+        int best = n;
+        if (!use_grid) {
+            for (int j = from; j < n; j++)
+                if (j != i && ref_pbc_dist(pi, &P.x[3 * (size_t)j], box) < 1.0 * rsum) return j;
+            return n;
+        }
+        const int G = grid.G;
+        const int cx = grid.c1(pi[0]), cy = grid.c1(pi[1]), cz = grid.c1(pi[2]);
+        for (int oz = -1; oz <= 1; ++oz)
+            for (int oy = -1; oy <= 1; ++oy)
+                for (int ox = -1; ox <= 1; ++ox) {
+                    const int c = (((cz + oz + G) % G) * G + (cy + oy + G) % G) * G + (cx + ox + G) % G;
+                    for (int j : grid.cells[c])
+                        if (j >= from && j < best && j != i &&
+                            ref_pbc_dist(pi, &P.x[3 * (size_t)j], box) < 1.0 * rsum)
+                            best = j;
+                }
+        return best;
+    };
+    long long counter = 0;
+    for (int i = seed_cube; i < n; i++) {
+        bool resolution = false;
+        while (!resolution) {
+            resolution = true;
+            int j = next_overlap(i, 0);
+            while (j < n) {
+                counter++;
+                if (counter > 10LL * n) return PSIM_ERR_INIT;   // throwInitializationError
+                resolution = false;
+                if (use_grid) grid.remove(i);
+                P.set(i, 0, distribution(gen) * box);
+                P.set(i, 1, distribution(gen) * box);
+                P.set(i, 2, distribution(gen) * box);
+                if (use_grid) grid.insert(i, &P.x[3 * (size_t)i]);
+                j = next_overlap(i, j + 1);
+            }
+        }
+    }
+    // maxwellVelocityInit (:161-267); the temperature is truncated to int
+    std::vector<double> v(3 * (size_t)n, 0.0);
+    const int itemp = (int)temp;
+    if (n == 1) {
+        for (int d = 0; d < 3; d++) {
+            const int dir = (distribution(gen) < 0.5) ? -1 : 1;
+            v[d] = dir * std::sqrt(itemp);
+        }
+    } else {
+        for (int d = 0; d < 3; d++)
+            for (int i = 0; i < n; i++) {
+                const double r1 = distribution(gen);
+                const double r2 = distribution(gen);
+                v[3 * (size_t)i + d] = std::sqrt(-2.0 * std::log(r1)) * std::cos(8.0 * std::atan(1) * r2);
+            }
+        for (int d = 0; d < 3; d++) {
+            double vsum = 0, vsum2 = 0;
+            for (int i = 0; i < n; i++) {
+                const double vx = v[3 * (size_t)i + d];
+                vsum = vsum + vx;
+                vsum2 = vsum2 + (vx * vx);
+            }
+            vsum = vsum / n;
+            vsum2 = vsum2 / n;
+            const double sigold = std::sqrt(vsum2 - (vsum * vsum));
+            const double vsig = std::sqrt(itemp);
+            const double ratio = vsig / sigold;
+            for (int i = 0; i < n; i++) v[3 * (size_t)i + d] = ratio * (v[3 * (size_t)i + d] - vsum);
+        }
+    }
+    std::copy(P.x.begin(), P.x.end(), pos_out);
+    if (pos0_out) std::copy(P.x0.begin(), P.x0.end(), pos0_out);
+    if (vel_out) std::copy(v.begin(), v.end(), vel_out);
+    return 0;
+}

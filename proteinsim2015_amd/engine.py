@@ -234,6 +234,20 @@ def _aos3(a, n, name):
     return a
 
 
+def reference_initial_state(n: int, box: int, radius: float = 0.5, seed: int = 90210,
+                            seed_size: int = 0, kT: float = 1.0) -> dict:
+    """The reference's own initial configuration (system::initParticles,
+    initCheck and maxwellVelocityInit, systemInit.cpp:31-267) for an integer
+    box: the same std::mt19937 draws in the same order, so the positions and
+    velocities equal those the reference would create with this seed."""
+    lib = _lib.load()
+    pos, pos0, vel = (np.empty(3 * n) for _ in range(3))
+    check(lib.psim_init_reference(int(n), int(box), float(radius), int(seed) & 0xFFFFFFFF,
+                                  int(seed_size), float(kT), _ptr(pos), _ptr(pos0), _ptr(vel)),
+          "psim_init_reference")
+    return {"pos": pos.reshape(n, 3), "pos0": pos0.reshape(n, 3), "vel": vel.reshape(n, 3)}
+
+
 def rsa_positions(n: int, box: float, seed: int, dmin: float = 1.0) -> np.ndarray:
     """Seeded random sequential addition (native, O(N)); error 7701 if the
     packing cannot be completed."""
