@@ -34,6 +34,10 @@
  *                          adaptor's batch path
  *   psim_gpu_get_state     particle getters (particle.h:82-240)
  *   psim_gpu_last_error    error::throw* (error.cpp:40-62), as a code
+ *   psim_write_recovery /  analysisManager::writeSystem / system::readParticles
+ *   psim_read_recovery     (analysisOutput.cpp:55-74, systemRecovery.cpp:30-81)
+ *   psim_write_xyz         analysisManager::writeSystemXYZ (analysisOutput.cpp:31-53)
+ *   psim_write_sys_config  system::writeSystemInit (system.cpp:108-127)
  *   psim_init_rsa          system::initParticles + initCheck
  *                          (systemInit.cpp:31-160), O(N) with a cell grid
  *   psim_init_reference    system::initParticles + initCheck +
@@ -77,7 +81,8 @@ enum {
     PSIM_E_ARG = -1,
     PSIM_E_HIP = -2,
     PSIM_E_NOMEM = -3,
-    PSIM_E_UNSUPPORTED = -4
+    PSIM_E_UNSUPPORTED = -4,
+    PSIM_E_IO = -5           /* file missing, unreadable or malformed     */
 };
 
 typedef struct psim_params {
@@ -142,6 +147,9 @@ int psim_gpu_get_state(psim_ctx *ctx, double *pos, double *pos0, double *vel,
 int psim_gpu_step(psim_ctx *ctx, int nsteps, const double *noise);
 /* As psim_gpu_step with noise already resident in device memory. */
 int psim_gpu_step_device_noise(psim_ctx *ctx, int nsteps, const double *d_noise);
+
+/* The parameters the context was created with (after psim_gpu_set_force_params). */
+int psim_gpu_get_params(psim_ctx *ctx, psim_params *out);
 
 /* Forces at the current positions (AoS3, original order); no state change. */
 int psim_gpu_forces(psim_ctx *ctx, double *force_out);
@@ -228,6 +236,12 @@ int psim_gpu_set_force_params(psim_ctx *ctx, const psim_params *params);
 /* Start the displacement tracker at the current positions: zero the
  * periodic-crossing counters, which every later step updates. */
 int psim_gpu_track_start(psim_ctx *ctx);
+
+/* The tracker's start positions and periodic-crossing counters (AoS3,
+ * original order; either may be NULL).  PSIM_E_UNSUPPORTED before
+ * psim_gpu_track_start.  set restores both (a resumed run). */
+int psim_gpu_get_tracker(psim_ctx *ctx, double *start, int *crossings);
+int psim_gpu_set_tracker(psim_ctx *ctx, const double *start, const int *crossings);
 
 typedef struct psim_observables {
     double temperature;       /* sum v^2 / 3 / N                             */
@@ -317,6 +331,39 @@ int psim_init_reference(int n, int box, double radius, unsigned int seed, int se
  * in a periodic cube [0, box)^3 (host, O(n)).  Returns 0 or PSIM_ERR_INIT. */
 int psim_init_rsa(int n, double box, double dmin, unsigned long long seed,
                   double *pos_out);
+
+/* ---- snapshots (host) ------------------------------------------------------ */
+/* analysisManager::writeSystem (analysisOutput.cpp:55-74): one line per
+ * particle, "x y z x0 y0 z0 fx fy fz fx0 fy0 fz0 m r", each number in the C++
+ * stream's default format (6 significant digits), no newline after the last
+ * line.  Arrays AoS3 in original order; frc / frc0 may be NULL (zeros). */
+int psim_write_recovery(const char *path, int n, const double *pos, const double *pos0,
+                        const double *frc, const double *frc0, double mass, double radius);
+/* system::particlesInFile + readParticles (systemRecovery.cpp:30-81): each
+ * line moves a fresh particle to its previous position and then to its
+ * current one through the particle setters (single wrap, previous position
+ * re-imaged next to the current one), so pos0 is the reference's; mass and
+ * radius are read as float.  The file must hold n lines (else PSIM_E_IO); a
+ * position outside [0, box) after the wrap gives PSIM_ERR_BOUNDS.  frc, frc0,
+ * mass[n], radius[n] may be NULL. */
+int psim_read_recovery(const char *path, int n, double box, double *pos, double *pos0,
+                       double *frc, double *frc0, double *mass, double *radius);
+/* analysisManager::writeSystemXYZ (analysisOutput.cpp:31-53): "N", "Current
+ * Time: <rounded time>", then "H x y z" per particle. */
+int psim_write_xyz(const char *path, int n, const double *pos, double current_time);
+/* system::writeSystemInit's sysConfig.cfg (system.cpp:108-127): the
+ * run's geometry and settings as "key = value" lines. */
+int psim_write_sys_config(const char *path, const char *trial_name, int n,
+                          double concentration, int box, int cell_size, int cell_scale,
+                          double temp, double dt, int output_freq, double cycle_hour, int seed);
+/* Full-precision state of a context (positions, previous positions,
+ * velocities, last forces, integrator memory, time, velocity counter, step
+ * index and, once started, the displacement tracker), so that a resumed run
+ * continues bit for bit.  No reference counterpart (the reference's
+ * recovery file drops velocities, memory, time and tracker).  load needs a
+ * context created with the same particle count and box (else PSIM_E_ARG). */
+int psim_gpu_save(psim_ctx *ctx, const char *path);
+int psim_gpu_load(psim_ctx *ctx, const char *path);
 
 #ifdef __cplusplus
 }

@@ -159,7 +159,10 @@ class Reference:
         reference's RecoverySystem; returns the directory holding the
         initialState.txt it writes for the loaded particles."""
         rewind = os.path.join(sys_state, f"-rewind-{stamp}")
-        os.makedirs(os.path.join(rewind, "movie"), exist_ok=True)
+        # RecoverySystem::createRewindDir asks on stdin before writing into an
+        # existing directory: the directory must not exist yet
+        if os.path.exists(rewind):
+            raise FileExistsError(rewind)
         path = write_cfg(cfg)
         try:
             st = self.lib.ref_recover(path.encode(), sys_state.encode(), stamp.encode())

@@ -15,6 +15,7 @@ ABI_VERSION = 1
 
 FORCE_CAL, FORCE_LJ, FORCE_AO = 1, 2, 4
 ERR_INIT, ERR_BOUNDS, ERR_OVERLAP, ERR_NAN = 7701, 7703, 7704, 7705
+E_ARG, E_HIP, E_NOMEM, E_UNSUPPORTED, E_IO = -1, -2, -3, -4, -5
 
 
 class PsimParams(C.Structure):
@@ -70,6 +71,9 @@ EXPORTS = (
     "psim_gpu_clusters", "psim_gpu_pair_histogram", "psim_gpu_pair_histogram_grad",
     "psim_adj_create", "psim_adj_destroy", "psim_adj_set_initial", "psim_adj_forward",
     "psim_adj_backward", "psim_init_reference",
+    "psim_gpu_get_params", "psim_gpu_get_tracker", "psim_gpu_set_tracker",
+    "psim_write_recovery", "psim_read_recovery", "psim_write_xyz", "psim_gpu_save",
+    "psim_gpu_load", "psim_write_sys_config",
 )
 
 _vp = C.c_void_p
@@ -123,6 +127,19 @@ _SIGS = {
     "psim_adj_backward": ([_vp, _dp, _dp, _dp], C.c_int),
     "psim_init_reference": ([C.c_int, C.c_int, C.c_double, C.c_uint, C.c_int, C.c_double, _dp,
                              _dp, _dp], C.c_int),
+    "psim_gpu_get_params": ([_vp, C.POINTER(PsimParams)], C.c_int),
+    "psim_gpu_get_tracker": ([_vp, _dp, _ip], C.c_int),
+    "psim_gpu_set_tracker": ([_vp, _dp, _ip], C.c_int),
+    "psim_write_recovery": ([C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double],
+                            C.c_int),
+    "psim_read_recovery": ([C.c_char_p, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp],
+                           C.c_int),
+    "psim_write_xyz": ([C.c_char_p, C.c_int, _dp, C.c_double], C.c_int),
+    "psim_gpu_save": ([_vp, C.c_char_p], C.c_int),
+    "psim_gpu_load": ([_vp, C.c_char_p], C.c_int),
+    "psim_write_sys_config": ([C.c_char_p, C.c_char_p, C.c_int, C.c_double, C.c_int, C.c_int,
+                              C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int],
+                             C.c_int),
 }
 
 _LIB = None

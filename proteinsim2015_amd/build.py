@@ -5,6 +5,8 @@
 Products (git-ignored, shipped with the tree to the GPU host):
   proteinsim2015_amd/libpsim_gpu.so        HIP kernels (gfx950) + C ABI
   proteinsim2015_amd/plugins/*.so          PSim::IForce plugin adaptors
+  proteinsim2015_amd/cli/psim_sim          the simulation program (settings.cfg
+                                           in, trial directory out)
 Test infrastructure built alongside (never loaded by the product):
   oracle/libpsim_oracle.so                 C restatement of the reference
   oracle/_ref/*                            the reference's own sources, when
@@ -61,7 +63,7 @@ def build_product(verbose: bool = False) -> str:
         obj = os.path.join(tmp, src + ".o")
         jobs.append((obj, [hipcc, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c",
                            os.path.join(CSRC, src), "-o", obj]))
-    for src in ("psim_host.cpp",):
+    for src in ("psim_host.cpp", "psim_io.cpp"):
         obj = os.path.join(tmp, src + ".o")
         jobs.append((obj, ["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-I",
                            INCLUDE, "-c", os.path.join(CSRC, src), "-o", obj]))
@@ -94,6 +96,11 @@ def build_plugins(verbose: bool = False) -> None:
     _run(["g++", "-std=c++17", "-O2", "-fopenmp", "-Wall", "-I", INCLUDE,
           os.path.join(PLUGIN_DIR, "psim_host_demo.cpp"), "-o", demo + ".tmp", "-ldl"], verbose)
     os.replace(demo + ".tmp", demo)
+    cli = os.path.join(PKG, "cli", "psim_sim")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE,
+          os.path.join(PKG, "cli", "psim_sim.cpp"), "-o", cli + ".tmp", "-L", PKG, "-lpsim_gpu",
+          "-Wl,-rpath,$ORIGIN/.."], verbose)
+    os.replace(cli + ".tmp", cli)
 
 
 def build_oracle(verbose: bool = False) -> None:

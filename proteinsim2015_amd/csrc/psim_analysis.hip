@@ -445,6 +445,50 @@ int psim_gpu_track_start(psim_ctx *c)
     return 0;
 }
 
+int psim_gpu_get_tracker(psim_ctx *c, double *start, int *crossings)
+{
+    if (!c || !c->have_state) return PSIM_E_ARG;
+    if (!c->pbc) return PSIM_E_UNSUPPORTED;   // tracker not started
+    const size_t n = (size_t)c->n;
+    std::vector<double> hs(3 * n);
+    std::vector<int> hp(3 * n);
+    CK(hipMemcpyAsync(hs.data(), c->track0, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                      c->stream));
+    CK(hipMemcpyAsync(hp.data(), c->pbc, sizeof(int) * 3 * n, hipMemcpyDeviceToHost,
+                      c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    for (size_t o = 0; o < n; ++o)
+        for (int d = 0; d < 3; ++d) {
+            if (start) start[3 * o + d] = hs[(size_t)d * n + o];
+            if (crossings) crossings[3 * o + d] = hp[(size_t)d * n + o];
+        }
+    return 0;
+}
+
+int psim_gpu_set_tracker(psim_ctx *c, const double *start, const int *crossings)
+{
+    if (!c || !c->have_state || !start || !crossings) return PSIM_E_ARG;
+    const size_t n = (size_t)c->n;
+    if (!c->pbc) {
+        int rc = dalloc(&c->pbc, 3 * n);
+        if (!rc) rc = dalloc(&c->track0, 3 * n);
+        if (rc) return rc;
+    }
+    std::vector<double> hs(3 * n);
+    std::vector<int> hp(3 * n);
+    for (size_t o = 0; o < n; ++o)
+        for (int d = 0; d < 3; ++d) {
+            hs[(size_t)d * n + o] = start[3 * o + d];
+            hp[(size_t)d * n + o] = crossings[3 * o + d];
+        }
+    CK(hipMemcpyAsync(c->track0, hs.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                      c->stream));
+    CK(hipMemcpyAsync(c->pbc, hp.data(), sizeof(int) * 3 * n, hipMemcpyHostToDevice,
+                      c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int psim_gpu_observables(psim_ctx *c, double contact_r, psim_observables *out,
                          int *coord_by_orig)
 {

@@ -263,6 +263,7 @@ const char *psim_gpu_status_string(int s)
     case PSIM_E_HIP: return "HIP runtime error";
     case PSIM_E_NOMEM: return "device allocation failed";
     case PSIM_E_UNSUPPORTED: return "unsupported configuration";
+    case PSIM_E_IO: return "file could not be read or written";
     default: return "unknown status";
     }
 }
@@ -328,6 +329,13 @@ int psim_gpu_create(const psim_params *params, psim_ctx **out)
         return rc;
     }
     *out = c;
+    return 0;
+}
+
+int psim_gpu_get_params(psim_ctx *c, psim_params *out)
+{
+    if (!c || !out) return PSIM_E_ARG;
+    *out = c->prm;
     return 0;
 }
 

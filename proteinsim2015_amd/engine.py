@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 import re
 from dataclasses import dataclass, field
 
@@ -332,6 +333,33 @@ class GpuSystem:
         st = {k: v.reshape(n, 3) for k, v in out.items()}
         st.update(time=t.value, vel_counter=vc.value, step_index=si.value)
         return st
+
+    def save(self, path: str) -> None:
+        """Full-precision state file (positions, previous positions,
+        velocities, last forces, integrator memory, time, counters and the
+        displacement tracker)."""
+        check(self.lib.psim_gpu_save(self._ctx, os.fsencode(path)), f"save {path}")
+
+    def load(self, path: str) -> None:
+        check(self.lib.psim_gpu_load(self._ctx, os.fsencode(path)), f"load {path}")
+
+    def get_tracker(self):
+        """(start positions, crossing counters), AoS3 by original index."""
+        start = np.empty(3 * self.n)
+        cross = np.empty(3 * self.n, dtype=np.int32)
+        check(self.lib.psim_gpu_get_tracker(self._ctx, _ptr(start),
+                                            cross.ctypes.data_as(C.POINTER(C.c_int))),
+              "psim_gpu_get_tracker")
+        return start.reshape(self.n, 3), cross.reshape(self.n, 3)
+
+    def set_tracker(self, start, crossings) -> None:
+        start = _aos3(start, self.n, "start")
+        cross = np.ascontiguousarray(crossings, dtype=np.int32).reshape(-1)
+        if cross.size != 3 * self.n:
+            raise ValueError("crossings: expected n x 3 ints")
+        check(self.lib.psim_gpu_set_tracker(self._ctx, _ptr(start),
+                                            cross.ctypes.data_as(C.POINTER(C.c_int))),
+              "psim_gpu_set_tracker")
 
     # -- hot path -------------------------------------------------------------
     def step(self, nsteps: int = 1, noise=None) -> int:

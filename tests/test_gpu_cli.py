@@ -1,0 +1,156 @@
+"""The simulation program (cli/psim_sim) on the GPU: a run, both resumes and
+the analysis mode, checked against the Python API driving the same engine
+and against the oracle (needs a GPU).
+
+* fresh run: every snapshot's recovery.txt equals (as text) the state the
+  Python API reaches after the same steps from the reference's initial
+  configuration; the series files hold the observables of those states;
+* -r DIR STAMP with state.psim: the resumed run's later snapshots equal the
+  uninterrupted run's, byte for byte (recovery.txt and state.psim);
+* -r DIR STAMP --from-recovery: the reference's resume (recovery.txt only,
+  velocities and memory zero, time restarted at 0);
+* -a DIR STAMP -CH -CLH -CLC: the histograms of the oracle's contact scan
+  and components on the loaded positions.
+"""
+import os
+import shutil
+import subprocess
+from collections import Counter
+
+import numpy as np
+import pytest
+
+import proteinsim2015_amd as ps
+from proteinsim2015_amd.engine import reference_initial_state
+from proteinsim2015_amd.snapshot import cli_path, read_recovery, write_recovery
+
+pytestmark = pytest.mark.gpu
+
+N, BOX, S = 300, 9, 3
+SETTINGS = """trialName = {trial}
+Integrator = brownianIntegrator
+nParticles = 300
+conc = 0.2
+scale = 3
+force = AO
+cutOff = 1.1
+wellDepth = 0.25
+endTime = 3
+timeStep = 0.01
+gamma = 0.5
+kT = 1
+seed = 90210
+radius = 0.5
+mass = 1
+velFreq = 40
+outputFreq = 100
+"""
+
+
+def run_cli(cwd, *args):
+    print("psim_sim", *args, flush=True)
+    r = subprocess.run([cli_path(), *args], cwd=cwd, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r
+
+
+@pytest.fixture(scope="module")
+def trial(tmp_path_factory):
+    base = tmp_path_factory.mktemp("cli")
+    (base / "settings.cfg").write_text(SETTINGS.format(trial=base / "trial"))
+    run_cli(base)
+    return base
+
+
+def api_system():
+    geo = ps.SystemGeometry(n=N, box=float(BOX), cells_per_dim=S, cell_size=3.0)
+    ao = ps.AOPotential(kT=1.0, well_depth=0.25, cutoff=1.1)
+    integ = ps.BrownianIntegrator(kT=1.0, mass=1.0, gamma=0.5, dt=0.01, vel_freq=40,
+                                  seed=90210)
+    return ps.GpuSystem(geo, ao, integ)
+
+
+def recovery_text(tmp, st, frc0):
+    p = str(tmp / "expected.txt")
+    write_recovery(p, st["pos"], st["pos0"], st["frc"], frc0, 1.0, 0.5)
+    return open(p).read()
+
+
+def series(path):
+    return {line.split()[0]: float(line.split()[1]) for line in open(path).read().splitlines()}
+
+
+def test_fresh_run_matches_the_api(trial, tmp_path):
+    t = trial / "trial"
+    for f in ("sysConfig.cfg", "initialState.txt", "settings.cfg", "movie/system-initial.xyz",
+              "movie/system-2.xyz", "snapshots/time-2/state.psim"):
+        assert (t / f).exists(), f
+    init = reference_initial_state(N, BOX, 0.5, 90210, 0, 1.0)
+    zeros = np.zeros((N, 3))
+    assert open(t / "initialState.txt").read() == recovery_text(
+        tmp_path, {"pos": init["pos"], "pos0": init["pos0"], "frc": zeros}, zeros)
+    coor, msd, disp = (series(t / f) for f in ("coorGraph.txt", "trackedMeanR2Graph.txt",
+                                               "meanR2Graph.txt"))
+    assert sorted(coor) == ["0", "1", "2"]
+    with api_system() as sysg:
+        sysg.set_state(init["pos"], init["pos0"], init["vel"])
+        sysg.track_start()
+        frc0 = zeros
+        done = 0
+        for stamp, at in (("0", 1), ("1", 101), ("2", 201)):
+            if at - 1 > done:
+                assert sysg.step(at - 1 - done) == 0
+                frc0 = sysg.get_state()["frc"]
+            assert sysg.step(1) == 0
+            done = at
+            st = sysg.get_state()
+            got = open(t / f"snapshots/time-{stamp}/recovery.txt").read()
+            assert got == recovery_text(tmp_path, st, frc0), stamp
+            ob = sysg.observables(1.2, coordination=True)
+            assert coor[stamp] == pytest.approx(ob["mean_coordination"], rel=1e-5)
+            assert msd[stamp] == pytest.approx(ob["msd"], rel=1e-5)
+            assert disp[stamp] == pytest.approx(ob["mean_disp2"], rel=1e-5)
+            frc0 = st["frc"]
+
+
+def test_exact_resume_continues_bit_for_bit(trial):
+    run_cli(trial, "-r", "trial", "1")
+    a, b = trial / "trial", trial / "trial" / "-rewind-1"
+    for f in ("recovery.txt", "state.psim"):
+        assert (b / "snapshots/time-2" / f).read_bytes() == \
+            (a / "snapshots/time-2" / f).read_bytes(), f
+    assert series(b / "trackedMeanR2Graph.txt")["2"] == series(a / "trackedMeanR2Graph.txt")["2"]
+
+
+def test_reference_resume_from_recovery(trial, tmp_path):
+    base = tmp_path / "copy"
+    shutil.copytree(trial / "trial", base / "trial")
+    shutil.copy(trial / "settings.cfg", base / "settings.cfg")
+    shutil.rmtree(base / "trial" / "-rewind-1", ignore_errors=True)
+    run_cli(base, "-r", "trial", "1", "--from-recovery")
+    rec = read_recovery(str(base / "trial/snapshots/time-1/recovery.txt"), N, BOX)
+    with api_system() as sysg:
+        sysg.set_state(rec["pos"], rec["pos0"], None, rec["frc"], time=0.0)
+        sysg.track_start()
+        assert sysg.step(1) == 0
+        st = sysg.get_state()
+    got = open(base / "trial/-rewind-1/snapshots/time-0/recovery.txt").read()
+    assert got == recovery_text(tmp_path, st, rec["frc"])
+
+
+def test_analysis_mode_histograms(trial, oracle):
+    run_cli(trial, "-a", "trial", "2", "-CH", "-CLH", "-CLC")
+    rec = read_recovery(str(trial / "trial/snapshots/time-2/recovery.txt"), N, BOX)
+    coord, lab = oracle.contacts(rec["pos"], S, 3.0, float(BOX), rc=1.2, labels=True)
+    sizes = Counter(lab.tolist())
+    big = {k for k, v in sizes.items() if v > 4}
+    want = {
+        "coordinationHistogram.txt": Counter(coord.tolist()),
+        "clusterSizeHistogram.txt": Counter(sizes[k] for k in big),
+        "clusterCoorHistogram.txt": Counter(int(coord[i]) for i in range(N) if lab[i] in big),
+    }
+    d = trial / "trial" / "-analysis-2"
+    for f, h in want.items():
+        text = "".join(f"{k} {h[k]}\n" for k in sorted(h))
+        assert open(d / f).read() == text, f
