@@ -358,7 +358,8 @@ int psim_write_sys_config(const char *path, const char *trial_name, int n,
                           double temp, double dt, int output_freq, double cycle_hour, int seed);
 /* Full-precision state of a context (positions, previous positions,
  * velocities, last forces, integrator memory, time, velocity counter, step
- * index and, once started, the displacement tracker), so that a resumed run
+ * index, the particles' slot order, which fixes the force summation order,
+ * and, once started, the displacement tracker), so that a resumed run
  * continues bit for bit.  No reference counterpart (the reference's
  * recovery file drops velocities, memory, time and tracker).  load needs a
  * context created with the same particle count and box (else PSIM_E_ARG). */

@@ -89,6 +89,10 @@ int launch_build(psim_ctx *c);
 // nsteps of system::run with the context's step bookkeeping
 int run_steps(psim_ctx *c, int nsteps, const double *d_noise);
 int read_error(psim_ctx *c, int *code, int *i, int *j);
+// psim_gpu_set_state with an explicit staging slot order (null = identity)
+int set_state(psim_ctx *c, const double *pos, const double *pos0, const double *vel,
+              const double *frc, const double *memx, double current_time, int vel_counter,
+              unsigned long long step_index, const int *orig_by_slot);
 
 }  // namespace eng
 }  // namespace psim

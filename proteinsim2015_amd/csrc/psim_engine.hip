@@ -234,6 +234,74 @@ int dalloc_bytes(void **p, size_t bytes)
     return 0;
 }
 
+// Upload a state (AoS3 by original index).  orig_by_slot (a permutation, or
+// null for ascending original index) is the slot order the particles are
+// staged in: the cell build keeps it within each cell, so a state saved with
+// its slot order continues with the same force summation order.
+int set_state(psim_ctx *c, const double *pos, const double *pos0, const double *vel,
+              const double *frc, const double *memx, double current_time, int vel_counter,
+              unsigned long long step_index, const int *orig_by_slot)
+{
+    if (!c || !pos) return PSIM_E_ARG;
+    const int n = c->n;
+    const double L = c->prm.box;
+    std::vector<int> ids(n);
+    if (orig_by_slot) {
+        std::vector<char> seen(n, 0);
+        for (int s = 0; s < n; ++s) {
+            const int o = orig_by_slot[s];
+            if (o < 0 || o >= n || seen[o]) return PSIM_E_ARG;
+            seen[o] = 1;
+            ids[s] = o;
+        }
+    } else {
+        for (int s = 0; s < n; ++s) ids[s] = s;
+    }
+    // host AoS3 (original order) -> SoA into the n* staging arrays
+    std::vector<double> h(3 * (size_t)n);
+    auto soa = [&](const double *src, double *dst, bool is_pos0, bool by_slot) -> int {
+        for (int s = 0; s < n; ++s)
+            for (int d = 0; d < 3; ++d) {
+                const int i = by_slot ? ids[s] : s;
+                double v;
+                if (src) {
+                    v = src[3 * (size_t)i + d];
+                } else if (is_pos0) {
+                    // a fresh particle: pos0 = safeMod0(0, x) (particle.cpp:31-66)
+                    double x = pos[3 * (size_t)i + d];
+                    double dx = x - 0.0;
+                    v = (std::fabs(dx) > (L / 2)) ? ((dx < 0) ? 0.0 - L : 0.0 + L) : 0.0;
+                } else {
+                    v = 0.0;
+                }
+                h[(size_t)d * n + s] = v;
+            }
+        CK(hipMemcpyAsync(dst, h.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                          c->stream));
+        CK(hipStreamSynchronize(c->stream));
+        return 0;
+    };
+    int rc;
+    if ((rc = soa(pos, c->npos, false, true))) return rc;
+    if ((rc = soa(pos0, c->npos0, true, true))) return rc;
+    if ((rc = soa(vel, c->vel, false, false))) return rc;   // velocities by original index
+    if ((rc = soa(frc, c->nfrc, false, true))) return rc;
+    if ((rc = soa(memx, c->nmem, false, true))) return rc;
+    CK(hipMemcpyAsync(c->nid, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    CK(hipMemsetAsync(c->count, 0, sizeof(int) * c->ncell, c->stream));
+    k_hash_count<<<blocks_for(n, 256), 256, 0, c->stream>>>(c->geo, c->npos, c->key,
+                                                            c->arank, c->count);
+    CK(hipGetLastError());
+    rc = launch_build(c);
+    if (rc) return rc;
+    CK(hipStreamSynchronize(c->stream));
+    c->time = current_time;
+    c->vel_counter = vel_counter;
+    c->step_index = step_index;
+    c->have_state = true;
+    return 0;
+}
+
 int launch_build(psim_ctx *c) { return ::launch_build(c); }
 int run_steps(psim_ctx *c, int nsteps, const double *d_noise)
 {
@@ -361,56 +429,11 @@ int psim_gpu_set_state(psim_ctx *c, const double *pos, const double *pos0,
                        const double *memx, double current_time, int vel_counter,
                        unsigned long long step_index)
 {
-    if (!c || !pos) return PSIM_E_ARG;
-    const int n = c->n;
-    const double L = c->prm.box;
-    // host AoS3 (original order) -> SoA into the n* staging arrays
-    std::vector<double> h(3 * (size_t)n);
-    auto soa = [&](const double *src, double *dst, bool is_pos0) -> int {
-        for (int i = 0; i < n; ++i)
-            for (int d = 0; d < 3; ++d) {
-                double v;
-                if (src) {
-                    v = src[3 * (size_t)i + d];
-                } else if (is_pos0) {
-                    // a fresh particle: pos0 = safeMod0(0, x) (particle.cpp:31-66)
-                    double x = pos[3 * (size_t)i + d];
-                    double dx = x - 0.0;
-                    v = (std::fabs(dx) > (L / 2)) ? ((dx < 0) ? 0.0 - L : 0.0 + L) : 0.0;
-                } else {
-                    v = 0.0;
-                }
-                h[(size_t)d * n + i] = v;
-            }
-        CK(hipMemcpyAsync(dst, h.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice,
-                          c->stream));
-        CK(hipStreamSynchronize(c->stream));
-        return 0;
-    };
-    int rc;
-    if ((rc = soa(pos, c->npos, false))) return rc;
-    if ((rc = soa(pos0, c->npos0, true))) return rc;
-    if ((rc = soa(vel, c->vel, false))) return rc;
     // frc holds the force of the previous step (the next push makes it frc0);
     // frc0 of the caller is not needed by the integrator after the push
     (void)frc0;
-    if ((rc = soa(frc, c->nfrc, false))) return rc;
-    if ((rc = soa(memx, c->nmem, false))) return rc;
-    std::vector<int> ids(n);
-    for (int i = 0; i < n; ++i) ids[i] = i;
-    CK(hipMemcpyAsync(c->nid, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
-    CK(hipMemsetAsync(c->count, 0, sizeof(int) * c->ncell, c->stream));
-    k_hash_count<<<blocks_for(n, 256), 256, 0, c->stream>>>(c->geo, c->npos, c->key,
-                                                            c->arank, c->count);
-    CK(hipGetLastError());
-    rc = launch_build(c);
-    if (rc) return rc;
-    CK(hipStreamSynchronize(c->stream));
-    c->time = current_time;
-    c->vel_counter = vel_counter;
-    c->step_index = step_index;
-    c->have_state = true;
-    return 0;
+    return psim::eng::set_state(c, pos, pos0, vel, frc, memx, current_time, vel_counter,
+                                step_index, nullptr);
 }
 
 int psim_gpu_get_state(psim_ctx *c, double *pos, double *pos0, double *vel, double *frc,

@@ -6,13 +6,9 @@
 //                        (systemRecovery.cpp:30-81)
 //   psim_write_xyz       analysisManager::writeSystemXYZ (analysisOutput.cpp:31-53)
 //   psim_write_sys_config system::writeSystemInit's sysConfig.cfg (system.cpp:108-127)
-//   psim_gpu_save/load   no reference counterpart: the reference's recovery
-//                        file keeps 6 significant digits and drops the
-//                        velocities, the integrator memory, the time and the
-//                        tracker, so its resume restarts from the first-step
-//                        branch; this file keeps all of them bit for bit.
 //
-// Host code over the public C ABI only.
+// Host code; the full-precision state file (psim_gpu_save / psim_gpu_load)
+// is in psim_state.hip.
 #include <cerrno>
 #include <cmath>
 #include <cstdint>
@@ -72,19 +68,6 @@ inline int set_coord(double &x, double &x0, double val, double box)
     x0 = (std::fabs(dx) > (box / 2)) ? ((dx < 0) ? old - box : old + box) : old;
     return (x < 0.0 || x >= box) ? PSIM_ERR_BOUNDS : 0;
 }
-
-constexpr char kMagic[8] = {'P', 'S', 'I', 'M', 'S', 'T', '0', '1'};
-
-struct StateHeader {
-    char magic[8];
-    int32_t n;
-    int32_t has_tracker;
-    double box;
-    double time;
-    uint64_t step_index;
-    int32_t vel_counter;
-    int32_t reserved;
-};
 
 }  // namespace
 
@@ -198,74 +181,6 @@ int psim_write_sys_config(const char *path, const char *trial_name, int n,
     put_g(s, cycle_hour);
     s += "\nseed = " + std::to_string(seed);
     return write_text(path, s);
-}
-
-int psim_gpu_save(psim_ctx *ctx, const char *path)
-{
-    if (!ctx || !path) return PSIM_E_ARG;
-    psim_params prm;
-    int rc = psim_gpu_get_params(ctx, &prm);
-    if (rc) return rc;
-    const size_t m = 3 * (size_t)prm.n_particles;
-    std::vector<double> arr(5 * m);
-    StateHeader h{};
-    std::memcpy(h.magic, kMagic, sizeof kMagic);
-    h.n = prm.n_particles;
-    h.box = prm.box;
-    unsigned long long si = 0;
-    int vc = 0;
-    rc = psim_gpu_get_state(ctx, arr.data(), arr.data() + m, arr.data() + 2 * m,
-                            arr.data() + 3 * m, nullptr, arr.data() + 4 * m, &h.time, &vc, &si);
-    if (rc) return rc;
-    h.step_index = si;
-    h.vel_counter = vc;
-    std::vector<double> start(m);
-    std::vector<int> cross(m);
-    rc = psim_gpu_get_tracker(ctx, start.data(), cross.data());
-    if (rc && rc != PSIM_E_UNSUPPORTED) return rc;
-    h.has_tracker = rc ? 0 : 1;
-    File out(std::fopen(path, "wb"));
-    if (!out.f) return PSIM_E_IO;
-    bool ok = std::fwrite(&h, sizeof h, 1, out.f) == 1 &&
-              std::fwrite(arr.data(), sizeof(double), arr.size(), out.f) == arr.size();
-    if (ok && h.has_tracker)
-        ok = std::fwrite(start.data(), sizeof(double), m, out.f) == m &&
-             std::fwrite(cross.data(), sizeof(int), m, out.f) == m;
-    const bool closed = std::fclose(out.f) == 0;
-    out.f = nullptr;
-    return (ok && closed) ? 0 : PSIM_E_IO;
-}
-
-int psim_gpu_load(psim_ctx *ctx, const char *path)
-{
-    if (!ctx || !path) return PSIM_E_ARG;
-    psim_params prm;
-    int rc = psim_gpu_get_params(ctx, &prm);
-    if (rc) return rc;
-    File in(std::fopen(path, "rb"));
-    if (!in.f) return PSIM_E_IO;
-    StateHeader h{};
-    if (std::fread(&h, sizeof h, 1, in.f) != 1 || std::memcmp(h.magic, kMagic, sizeof kMagic))
-        return PSIM_E_IO;
-    if (h.n != prm.n_particles || h.box != prm.box) return PSIM_E_ARG;
-    const size_t m = 3 * (size_t)h.n;
-    std::vector<double> arr(5 * m);
-    if (std::fread(arr.data(), sizeof(double), arr.size(), in.f) != arr.size()) return PSIM_E_IO;
-    std::vector<double> start;
-    std::vector<int> cross;
-    if (h.has_tracker) {
-        start.resize(m);
-        cross.resize(m);
-        if (std::fread(start.data(), sizeof(double), m, in.f) != m ||
-            std::fread(cross.data(), sizeof(int), m, in.f) != m)
-            return PSIM_E_IO;
-    }
-    rc = psim_gpu_set_state(ctx, arr.data(), arr.data() + m, arr.data() + 2 * m,
-                            arr.data() + 3 * m, nullptr, arr.data() + 4 * m, h.time,
-                            h.vel_counter, h.step_index);
-    if (rc) return rc;
-    if (h.has_tracker) rc = psim_gpu_set_tracker(ctx, start.data(), cross.data());
-    return rc;
 }
 
 }  // extern "C"

@@ -146,7 +146,7 @@ def fit(problem: AOFitProblem, theta0, target_g: np.ndarray, iters: int = 20,
         theta = z * sc
         loss = gofr_loss(problem, theta, target)
         loss.backward()
-        hist.append((theta.detach().tolist(), float(loss), (z.grad / sc).tolist()))
+        hist.append((theta.detach().tolist(), loss.item(), (z.grad / sc).tolist()))
         opt.step()
         with torch.no_grad():
             z[0].clamp_(0.5 / scale[0], 12.0 / scale[0])

@@ -36,14 +36,14 @@ force = AO
 cutOff = 1.1
 wellDepth = 0.25
 endTime = 3
-timeStep = 0.01
+timeStep = 0.001
 gamma = 0.5
 kT = 1
 seed = 90210
 radius = 0.5
 mass = 1
 velFreq = 40
-outputFreq = 100
+outputFreq = 1000
 """
 
 
@@ -66,7 +66,7 @@ def trial(tmp_path_factory):
 def api_system():
     geo = ps.SystemGeometry(n=N, box=float(BOX), cells_per_dim=S, cell_size=3.0)
     ao = ps.AOPotential(kT=1.0, well_depth=0.25, cutoff=1.1)
-    integ = ps.BrownianIntegrator(kT=1.0, mass=1.0, gamma=0.5, dt=0.01, vel_freq=40,
+    integ = ps.BrownianIntegrator(kT=1.0, mass=1.0, gamma=0.5, dt=0.001, vel_freq=40,
                                   seed=90210)
     return ps.GpuSystem(geo, ao, integ)
 
@@ -92,13 +92,21 @@ def test_fresh_run_matches_the_api(trial, tmp_path):
         tmp_path, {"pos": init["pos"], "pos0": init["pos0"], "frc": zeros}, zeros)
     coor, msd, disp = (series(t / f) for f in ("coorGraph.txt", "trackedMeanR2Graph.txt",
                                                "meanR2Graph.txt"))
-    assert sorted(coor) == ["0", "1", "2"]
+    # system::run's loop (system.cpp:187-211) runs while the accumulated time
+    # is below endTime: 3000 additions of 0.001 fall short of 3, so a
+    # 3001st step runs and writes a snapshot at time 3
+    stamps, now, c = [], 0.0, 0
+    while now < 3:
+        if c % 1000 == 0:
+            stamps.append(str(int(np.round(now))))
+        c, now = c + 1, now + 0.001
+    assert sorted(coor) == stamps == ["0", "1", "2", "3"]
     with api_system() as sysg:
         sysg.set_state(init["pos"], init["pos0"], init["vel"])
         sysg.track_start()
         frc0 = zeros
         done = 0
-        for stamp, at in (("0", 1), ("1", 101), ("2", 201)):
+        for stamp, at in (("0", 1), ("1", 1001), ("2", 2001)):
             if at - 1 > done:
                 assert sysg.step(at - 1 - done) == 0
                 frc0 = sysg.get_state()["frc"]
