@@ -17,6 +17,8 @@ bracketed by stream synchronisations.
 from __future__ import annotations
 
 import argparse
+import contextlib
+import ctypes
 import json
 import math
 import os
@@ -86,6 +88,24 @@ def calibrate_copy():
     hip.hipFree(a)
     hip.hipFree(b)
     print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": 3}))
+
+
+@contextlib.contextmanager
+def stdout_to_stderr():
+    """Send what native code prints to stdout (the reference's console
+    messages) to stderr, so that stdout carries only the JSON line."""
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    libc.fflush(None)
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        yield
+    finally:
+        libc.fflush(None)
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
 
 
 def cpu_baseline(pos_exact, box_exact, steps):
@@ -223,7 +243,10 @@ def main():
         adjoint = bench_adjoint.run()
         print(f"bench: adjoint {adjoint['adjoint_ms_per_step']:.3f} ms per step",
               file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(pos, geom.box, args.cpu_steps)
+    cpu = None
+    if not args.no_cpu_baseline:
+        with stdout_to_stderr():
+            cpu = cpu_baseline(pos, geom.box, args.cpu_steps)
     line = {
         "metric": "particle-steps/s at N=2M",
         "value": value,
