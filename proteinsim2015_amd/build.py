@@ -36,7 +36,7 @@ ARCH = os.environ.get("PSIM_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-result", "-Wno-unused-function",
+    "-Wall", "-Wshadow", "-Wno-unused-result", "-Wno-unused-function",
 ]
 
 

@@ -157,10 +157,10 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
             constexpr int kPer = T / 27;
             if (lane < 27 * kPer) {
                 const int q = lane / kPer, u = lane % kPer;
-                const int lo = max(sh.coff[q], s0), hi = min(sh.coff[q + 1], s0 + mc);
+                const int t_lo = max(sh.coff[q], s0), t_hi = min(sh.coff[q + 1], s0 + mc);
                 const int base = sh.cbase[q] - sh.coff[q];
                 const double ex = sh.shx[q], ey = sh.shy[q], ez = sh.shz[q];
-                for (int t = lo + u; t < hi; t += kPer) {
+                for (int t = t_lo + u; t < t_hi; t += kPer) {
                     const int slot = base + t, k = t - s0;
                     sh.sx[k] = pos[slot] + ex;
                     sh.sy[k] = pos[N + slot] + ey;
