@@ -148,6 +148,41 @@ def cpu_baseline(pos_exact, box_exact, steps):
             "kind": kind, "sample": sample, "seconds": secs}
 
 
+def cpu_adjoint_baseline(steps=20):
+    """The oracle's FP64 CPU trajectory adjoint (forward with stored states,
+    then backward) on the adjoint workload's start, a bounded sample of the
+    K = 200 trajectory.  The reference has no derivatives, so the CPU side
+    is the oracle's restatement ("port")."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    import proteinsim2015_amd as ps
+    from proteinsim2015_amd import fit
+    from oracle import ForceParams, load_oracle
+    from oracle.oracle import PO_AO
+    o = load_oracle()
+    n = 32768
+    geom = ps.SystemGeometry.exact(n, 0.15, 1.2)
+    pos = ps.rsa_positions(n, geom.box, 1234)
+    vel = np.random.default_rng(1235).standard_normal((n, 3))
+    L = geom.box
+    pos0 = np.where(np.abs(pos) > L / 2, np.where(pos < 0, -L, L), 0.0)
+    phi_p, cut, _, _ = fit.ao_params(4.0, 0.10)
+    fp = ForceParams.make(PO_AO, cut, kT=1.0, ao_well_depth=phi_p)
+    c = o.gb_coeffs(1.0, 1.0, 1.0, 1e-3)
+    gx = np.random.default_rng(3).standard_normal((n, 3))
+    t0 = time.perf_counter()
+    st, _, _, _ = o.adjoint(fp, c, dict(pos=pos, pos0=pos0, vel=vel), steps, L,
+                            geom.cells_per_dim, geom.cell_size, gx, vel_freq=1000, seed=90210,
+                            nthreads=threads)
+    secs = time.perf_counter() - t0
+    if st:
+        raise RuntimeError(f"CPU adjoint failed ({st})")
+    return {"ms_per_step": secs / steps * 1e3, "unit": "ms per (forward + backward) step",
+            "cores": threads, "kind": "port",
+            "sample": f"N={n}, {steps} steps of the adjoint workload's trajectory, "
+                      f"OpenMP threads={threads}", "seconds": secs}
+
+
 def traffic_from_profiles():
     """HBM bytes per k_step launch measured by rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE) and calibrated against a copy kernel; null when absent."""
@@ -247,6 +282,13 @@ def main():
     if not args.no_cpu_baseline:
         with stdout_to_stderr():
             cpu = cpu_baseline(pos, geom.box, args.cpu_steps)
+        if adjoint is not None:
+            acpu = cpu_adjoint_baseline()
+            adjoint["cpu_baseline"] = acpu
+            adjoint["gpu_ms_per_forward_plus_backward_step"] = \
+                adjoint["forward_ms_per_step"] + adjoint["adjoint_ms_per_step"]
+            print(f"bench: CPU adjoint {acpu['ms_per_step']:.1f} ms per step "
+                  f"({acpu['cores']} threads)", file=sys.stderr, flush=True)
     line = {
         "metric": "particle-steps/s at N=2M",
         "value": value,
