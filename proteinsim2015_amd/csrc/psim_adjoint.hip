@@ -41,7 +41,7 @@ using eng::dalloc;
 namespace {
 
 constexpr int kAdjBlock = 256;
-constexpr int kAdjSmall = 256 * 1024;   // below this N, 64-lane blocks fill the CUs
+constexpr int kAdjSmall = 256 * 1024;   // below this N, 8 lanes per particle
 constexpr int kNTheta = 5;   // AO wellDepth, AO cutOff, LJ wellDepth, yukawaStrength, debyeLength
 
 // Host-side derivatives of the pair constants (AOPotential.cpp:50-55).
@@ -114,25 +114,27 @@ __device__ __forceinline__ void pair_fnet_d(const Pot &p, const AdjPot &q, doubl
 }
 
 // Pairs of slot s inside the cutoff, with the forward kernels' neighbour
-// test and displacement (psim_pairs.hpp gather_force / drain_hits).
-template <bool EXACT, class Fn>
+// test and displacement (psim_pairs.hpp gather_force / drain_hits).  Lane
+// `sub` of the SPLIT lanes that share the particle visits neighbour cells
+// sub, sub + SPLIT, ... of the 27.
+template <bool EXACT, int SPLIT, class Fn>
 __device__ void visit_pairs(const Geo &g, const Pot &p, const double *pos, const int *cstart,
-                            int s, Fn fn)
+                            int s, int sub, Fn fn)
 {
     const int N = g.n, S = g.S;
     const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
     const int cx = cell_1d(xi, g.cell, S), cy = cell_1d(yi, g.cell, S),
               cz = cell_1d(zi, g.cell, S);
-    for (int ox = -1; ox <= 1; ++ox) {
-        int rx = cx + ox, kx = 0;
-        if (rx < 0) { rx += S; kx = 1; } else if (rx >= S) { rx -= S; kx = 2; }
-        const double xs = shifted(xi, kx, g.L);
-        for (int oy = -1; oy <= 1; ++oy) {
-            int ry = cy + oy, ky = 0;
+    for (int q = sub; q < 27; q += SPLIT) {
+        {
+            int rx = cx + q / 9 - 1, kx = 0;
+            if (rx < 0) { rx += S; kx = 1; } else if (rx >= S) { rx -= S; kx = 2; }
+            const double xs = shifted(xi, kx, g.L);
+            int ry = cy + (q / 3) % 3 - 1, ky = 0;
             if (ry < 0) { ry += S; ky = 1; } else if (ry >= S) { ry -= S; ky = 2; }
             const double ys = shifted(yi, ky, g.L);
-            for (int oz = -1; oz <= 1; ++oz) {
-                int rz = cz + oz, kz = 0;
+            {
+                int rz = cz + q % 3 - 1, kz = 0;
                 if (rz < 0) { rz += S; kz = 1; } else if (rz >= S) { rz -= S; kz = 2; }
                 const double zs = shifted(zi, kz, g.L);
                 const int h = g.c0 + rx + S * ry + g.S2 * rz;
@@ -167,23 +169,25 @@ __device__ void visit_pairs(const Geo &g, const Pot &p, const double *pos, const
     }
 }
 
-// One backward step at the recorded state x_n.
-template <int MASK, bool EXACT, int BLOCK>
+// One backward step at the recorded state x_n (SPLIT lanes per particle).
+template <int MASK, bool EXACT, int BLOCK, int SPLIT>
 __global__ __launch_bounds__(BLOCK) void k_adj_step(
     Geo g, Pot p, AdjPot q, double radius, const double *pos, const int *id,
     const int *cstart, const double *lam, const double *a1, const double *a2, double *a0,
     double *lam_prev, double beta, double coeff0, double alpha_prev, double c2, double *tpart)
 {
-    const int s = blockIdx.x * BLOCK + threadIdx.x;
+    const int gt = blockIdx.x * BLOCK + threadIdx.x;
+    const int s = gt / SPLIT, sub = gt % SPLIT;
     const int N = g.n;
     double th[kNTheta] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (s < N) {
+    double jx = 0.0, jy = 0.0, jz = 0.0;
+    if (s < N) {   // whole lane groups
         const int o = id[s];
         const double lx = lam[o], ly = lam[N + o], lz = lam[2 * N + o];
         const double size = 2.0 * radius;
-        double jx = 0.0, jy = 0.0, jz = 0.0;
-        visit_pairs<EXACT>(g, p, pos, cstart, s,
-                           [&](int j, double dx, double dy, double dz, double r, double rinv) {
+        visit_pairs<EXACT, SPLIT>(g, p, pos, cstart, s, sub,
+                                  [&](int j, double dx, double dy, double dz, double r,
+                                      double rinv) {
             const int oj = id[j];
             const double ex = lam[oj] - lx, ey = lam[N + oj] - ly, ez = lam[2 * N + oj] - lz;
             double f, fp, dth[kNTheta];
@@ -199,6 +203,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_step(
 #pragma unroll
             for (int k = 0; k < kNTheta; ++k) th[k] += dth[k] * ld;
         });
+#pragma unroll
+        for (int d = 1; d < SPLIT; d <<= 1) {
+            jx += __shfl_xor(jx, d, 64);
+            jy += __shfl_xor(jy, d, 64);
+            jz += __shfl_xor(jz, d, 64);
+        }
+    }
+    if (s < N && sub == 0) {
+        const int o = id[s];
         const double n0x = beta * a1[o] - coeff0 * a2[o] + jx;
         const double n0y = beta * a1[N + o] - coeff0 * a2[N + o] + jy;
         const double n0z = beta * a1[2 * N + o] - coeff0 * a2[2 * N + o] + jz;
@@ -247,6 +260,28 @@ __global__ __launch_bounds__(256) void k_theta_sum(const double *tpart, long lon
     if (t == 0) acc[k] = red[0];
 }
 
+// Several device-to-device copies in one launch (the state arrays of a
+// checkpoint or of a recorded step): segment k copies words[k] 4-byte words.
+constexpr int kMaxSeg = 8;
+struct CopySegs {
+    const unsigned *src[kMaxSeg];
+    unsigned *dst[kMaxSeg];
+    long long end[kMaxSeg];   // cumulative word counts
+    int nseg;
+};
+
+__global__ __launch_bounds__(256) void k_copy_segs(CopySegs cs)
+{
+    const long long total = cs.end[cs.nseg - 1];
+    for (long long w = blockIdx.x * 256LL + threadIdx.x; w < total;
+         w += (long long)gridDim.x * 256) {
+        int k = 0;
+        while (w >= cs.end[k]) ++k;
+        const long long off = w - (k ? cs.end[k - 1] : 0);
+        cs.dst[k][off] = cs.src[k][off];
+    }
+}
+
 // lam = alpha * a1 (the last step: a_{K+1} = 0)
 __global__ void k_scale(int n3, double alpha, const double *a, double *out)
 {
@@ -282,6 +317,25 @@ namespace {
 size_t cp_stride_d(const psim_ctx *c) { return 15 * (size_t)c->n; }
 size_t cp_stride_i(const psim_ctx *c) { return (size_t)c->n + c->ncell + 1; }
 
+// Launch the copies collected in cs (one kernel; the grid covers the
+// largest system in a few passes).
+int copy_segs(psim_ctx *c, CopySegs &cs)
+{
+    const long long total = cs.end[cs.nseg - 1];
+    const long long blocks = (total + 255) / 256;
+    k_copy_segs<<<(unsigned)(blocks < 4096 ? blocks : 4096), 256, 0, c->stream>>>(cs);
+    CK(hipGetLastError());
+    return 0;
+}
+
+void add_seg(CopySegs &cs, const void *src, void *dst, size_t bytes)
+{
+    cs.src[cs.nseg] = (const unsigned *)src;
+    cs.dst[cs.nseg] = (unsigned *)dst;
+    cs.end[cs.nseg] = (cs.nseg ? cs.end[cs.nseg - 1] : 0) + (long long)(bytes / 4);
+    ++cs.nseg;
+}
+
 int save_cp(psim_adjoint *A, int k)
 {
     psim_ctx *c = A->c;
@@ -289,14 +343,12 @@ int save_cp(psim_adjoint *A, int k)
     double *d = A->cp_d + (size_t)k * cp_stride_d(c);
     int *ip = A->cp_i + (size_t)k * cp_stride_i(c);
     const double *src[5] = {c->pos, c->pos0, c->frc, c->mem, c->vel};
-    for (int q = 0; q < 5; ++q)
-        CK(hipMemcpyAsync(d + q * n3, src[q], sizeof(double) * n3, hipMemcpyDeviceToDevice,
-                          c->stream));
-    CK(hipMemcpyAsync(ip, c->id, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
-    CK(hipMemcpyAsync(ip + c->n, c->cstart, sizeof(int) * (c->ncell + 1),
-                      hipMemcpyDeviceToDevice, c->stream));
+    CopySegs cs{};
+    for (int q = 0; q < 5; ++q) add_seg(cs, src[q], d + q * n3, sizeof(double) * n3);
+    add_seg(cs, c->id, ip, sizeof(int) * c->n);
+    add_seg(cs, c->cstart, ip + c->n, sizeof(int) * (c->ncell + 1));
     A->meta[k] = {c->time, c->vel_counter, c->step_index};
-    return 0;
+    return copy_segs(c, cs);
 }
 
 int restore_cp(psim_adjoint *A, int k)
@@ -306,12 +358,12 @@ int restore_cp(psim_adjoint *A, int k)
     const double *d = A->cp_d + (size_t)k * cp_stride_d(c);
     const int *ip = A->cp_i + (size_t)k * cp_stride_i(c);
     double *dst[5] = {c->pos, c->pos0, c->frc, c->mem, c->vel};
-    for (int q = 0; q < 5; ++q)
-        CK(hipMemcpyAsync(dst[q], d + q * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice,
-                          c->stream));
-    CK(hipMemcpyAsync(c->id, ip, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
-    CK(hipMemcpyAsync(c->cstart, ip + c->n, sizeof(int) * (c->ncell + 1),
-                      hipMemcpyDeviceToDevice, c->stream));
+    CopySegs cs{};
+    for (int q = 0; q < 5; ++q) add_seg(cs, d + q * n3, dst[q], sizeof(double) * n3);
+    add_seg(cs, ip, c->id, sizeof(int) * c->n);
+    add_seg(cs, ip + c->n, c->cstart, sizeof(int) * (c->ncell + 1));
+    int rc = copy_segs(c, cs);
+    if (rc) return rc;
     c->time = A->meta[k].time;
     c->vel_counter = A->meta[k].vel_counter;
     c->step_index = A->meta[k].step;
@@ -334,7 +386,17 @@ bool same_forces(const psim_params &a, const psim_params &b)
            a.debye_length == b.debye_length && a.ao_well_depth == b.ao_well_depth;
 }
 
-int adj_block(const psim_ctx *c) { return c->n < kAdjSmall ? 64 : kAdjBlock; }
+int adj_block(const psim_ctx *) { return kAdjBlock; }
+// lanes per particle: small systems spread each particle's 27 cells over 8
+// lanes, so that every CU gets work and each lane waits on fewer lookups
+#ifndef PSIM_ADJ_SPLIT
+#define PSIM_ADJ_SPLIT 8
+#endif
+int adj_split(const psim_ctx *c) { return c->n < kAdjSmall ? PSIM_ADJ_SPLIT : 1; }
+unsigned adj_blocks(const psim_ctx *c)
+{
+    return blocks_for((long long)c->n * adj_split(c), adj_block(c));
+}
 
 // one backward step; its block partials of dL/dtheta go to tpart at tp
 template <int MASK>
@@ -343,16 +405,18 @@ void launch_adj_mask(psim_adjoint *A, const AdjPot &q, const double *pos, const 
                      double *a0, double *lam_prev, double beta, double alpha_prev, double *tp)
 {
     psim_ctx *c = A->c;
-    const int B = adj_block(c);
-    const unsigned nb = blocks_for(c->n, B);
-#define PSIM_ADJ_LAUNCH(EX, BL)                                                            \
-    k_adj_step<MASK, EX, BL><<<nb, BL, 0, c->stream>>>(                                    \
+    const unsigned nb = adj_blocks(c);
+#define PSIM_ADJ_LAUNCH(EX, BL, SP)                                                        \
+    k_adj_step<MASK, EX, BL, SP><<<nb, BL, 0, c->stream>>>(                                \
         c->geo, c->pot, q, c->prm.radius, pos, id, cs, lam, a1, a2, a0, lam_prev, beta,    \
         c->gb.coeff0, alpha_prev, c->gb.c2, tp)
+    const bool split = adj_split(c) > 1;
     if (c->geo.exact_mi) {
-        if (B == 64) PSIM_ADJ_LAUNCH(true, 64); else PSIM_ADJ_LAUNCH(true, kAdjBlock);
+        if (split) PSIM_ADJ_LAUNCH(true, kAdjBlock, PSIM_ADJ_SPLIT);
+        else PSIM_ADJ_LAUNCH(true, kAdjBlock, 1);
     } else {
-        if (B == 64) PSIM_ADJ_LAUNCH(false, 64); else PSIM_ADJ_LAUNCH(false, kAdjBlock);
+        if (split) PSIM_ADJ_LAUNCH(false, kAdjBlock, PSIM_ADJ_SPLIT);
+        else PSIM_ADJ_LAUNCH(false, kAdjBlock, 1);
     }
 #undef PSIM_ADJ_LAUNCH
 }
@@ -375,7 +439,7 @@ int psim_adj_create(psim_ctx *c, int nsteps, int checkpoint_every, psim_adjoint 
     A->first.assign(nsteps, 0);
     const size_t n3 = 3 * (size_t)c->n;
     // block partials of dL/dtheta for every step, summed once at the end
-    const size_t nb = blocks_for(c->n, adj_block(c));
+    const size_t nb = adj_blocks(c);
     int rc = 0;
     if (!rc) rc = dalloc(&A->cp_d, (size_t)(A->ncp + 1) * cp_stride_d(c));
     if (!rc) rc = dalloc(&A->cp_i, (size_t)(A->ncp + 1) * cp_stride_i(c));
@@ -466,7 +530,7 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
         CK(hipStreamSynchronize(c->stream));
     }
     CK(hipMemsetAsync(A->a[2], 0, sizeof(double) * n3, c->stream));
-    const size_t nbk = blocks_for(n, adj_block(c));
+    const size_t nbk = adj_blocks(c);
     k_scale<<<blocks_for((long long)n3, 256), 256, 0, c->stream>>>((int)n3, alpha(A->K - 1),
                                                                    A->a[1], A->lam[0]);
     CK(hipGetLastError());
@@ -480,13 +544,11 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
         if ((rc = restore_cp(A, k))) return rc;
         for (int s = n0; s < n1; ++s) {
             const int r = s - n0;
-            CK(hipMemcpyAsync(A->rec_pos + (size_t)r * n3, c->pos, sizeof(double) * n3,
-                              hipMemcpyDeviceToDevice, c->stream));
-            CK(hipMemcpyAsync(A->rec_i + (size_t)r * si, c->id, sizeof(int) * n,
-                              hipMemcpyDeviceToDevice, c->stream));
-            CK(hipMemcpyAsync(A->rec_i + (size_t)r * si + n, c->cstart,
-                              sizeof(int) * (c->ncell + 1), hipMemcpyDeviceToDevice,
-                              c->stream));
+            CopySegs cs{};
+            add_seg(cs, c->pos, A->rec_pos + (size_t)r * n3, sizeof(double) * n3);
+            add_seg(cs, c->id, A->rec_i + (size_t)r * si, sizeof(int) * n);
+            add_seg(cs, c->cstart, A->rec_i + (size_t)r * si + n, sizeof(int) * (c->ncell + 1));
+            if ((rc = copy_segs(c, cs))) return rc;
             if (s + 1 < n1 && (rc = eng::run_steps(c, 1, nullptr))) return rc;
         }
         for (int s = n1 - 1; s >= n0; --s) {

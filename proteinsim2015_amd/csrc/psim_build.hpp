@@ -32,7 +32,8 @@ __global__ void k_hash_count(Geo g, const double *pos, int *key, int *arank, int
     arank[s] = wave_aggregated_rank(count, k);
 }
 
-// Exclusive scan over a 1024-thread block of one value per thread.
+// Exclusive scan over a 1024-thread block of one value per thread; *total
+// receives the block's sum.
 __device__ int block_exclusive_scan(int v, int *total)
 {
     __shared__ int wsum[kScanThreads / 64];
@@ -74,23 +75,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int *count, 
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_bsums(int *bsum, int nb)
-{
-    int carry = 0;
-    for (int base = 0; base < nb; base += kScanThreads) {
-        int i = base + threadIdx.x;
-        int v = i < nb ? bsum[i] : 0;
-        int total;
-        int pre = block_exclusive_scan(v, &total);
-        if (i < nb) bsum[i] = carry + pre;
-        carry += total;
-    }
-}
-
+// Cell start table from the counts: each block adds the totals of the tiles
+// before it (bsum, written by k_scan_reduce) itself, then scans its tile.
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(int *count, int ncell,
                                                              const int *bsum,
                                                              int *cstart, int n)
 {
+    int before = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) before += bsum[b];
+    int before_total;
+    block_exclusive_scan(before, &before_total);
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int c[kScanItems];
     int v = 0;
@@ -100,7 +94,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(int *count, int nce
         v += c[k];
     }
     int total;
-    int pre = block_exclusive_scan(v, &total) + bsum[blockIdx.x];
+    int pre = block_exclusive_scan(v, &total) + before_total;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         if (base + k < ncell) {

@@ -84,7 +84,6 @@ int launch_build(psim_ctx *c)
     const int n = c->n, nc = c->ncell;
     hipStream_t st = c->stream;
     k_scan_reduce<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->bsum);
-    k_scan_bsums<<<1, kScanThreads, 0, st>>>(c->bsum, c->nscan);
     k_scan_apply<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->bsum, c->cstart, n);
     k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, c->key, c->arank, c->cstart, c->perm);
     k_move<<<blocks_for(n, 256), 256, 0, st>>>(n, c->perm, c->key, c->cstart, c->npos,
@@ -102,9 +101,11 @@ void launch_step_mask(psim_ctx *c, const StepArgs &a)
         k_step_dense<MASK><<<8 * per, kDenseThreads, 0, c->stream>>>(
             c->geo, c->pot, c->prm.radius, a.pos, a.id, a.cstart, a.nfrc, a.err, per);
     } else if (c->n < kSmallSystem) {
-        // few particles: small blocks so that every CU gets work
-        k_step<MASK, 64><<<blocks_for(c->n, 64), 64, 0, c->stream>>>(c->geo, c->pot, c->gb,
-                                                                     c->prm.radius, a);
+        // few particles: small blocks and kSmallSplit lanes per particle so
+        // that every CU gets work
+        k_step<MASK, 64, kSmallSplit>
+            <<<blocks_for((long long)c->n * kSmallSplit, 64), 64, 0, c->stream>>>(
+                c->geo, c->pot, c->gb, c->prm.radius, a);
     } else {
         k_step<MASK, kBlock><<<blocks_for(c->n, kBlock), kBlock, 0, c->stream>>>(
             c->geo, c->pot, c->gb, c->prm.radius, a);

@@ -192,7 +192,6 @@ void launch_build_ens(psim_ens *e)
     const int n = e->ntot, nc = e->ncell;
     hipStream_t st = e->stream;
     k_scan_reduce<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->bsum);
-    k_scan_bsums<<<1, kScanThreads, 0, st>>>(e->bsum, e->nscan);
     k_scan_apply<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->bsum, e->cstart, n);
     k_scatter<<<eblocks(n, 256), 256, 0, st>>>(n, e->key, e->arank, e->cstart, e->perm);
     k_move<<<eblocks(n, 256), 256, 0, st>>>(n, e->perm, e->key, e->cstart, e->npos, e->npos0,

@@ -130,20 +130,77 @@ __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
     }
 }
 
+// Candidates of one neighbour cell h (image code `code`) appended to the
+// lane's hit list.
+template <int MASK, bool EXACT, int BLOCK, class Src>
+__device__ __forceinline__ void cull_cell(const Geo &g, const Pot &p, const Src &src, int s,
+                                          double xi, double yi, double zi, double xs,
+                                          double ys, double zs, int b, int e, unsigned code,
+                                          unsigned *hits, int &nh, double ri, double &fx,
+                                          double &fy, double &fz, DevErr *err,
+                                          const Orig &orig)
+{
+    for (int j = b; j < e; ++j) {
+        bool hit;
+        if (!EXACT) {
+            double dx = src.x(j) - xs;
+            double dy = src.y(j) - ys;
+            double dz = src.z(j) - zs;
+            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            hit = (r2 < p.rc2_hi) && (j != s);
+        } else {
+            double r2 = pbc_dist_ref(xi, yi, zi, src.x(j), src.y(j), src.z(j), g.L,
+                                     g.half_pbc);
+            hit = (r2 < p.rc2) && (j != s);
+        }
+        if (hit) {
+            hits[nh * BLOCK] = (unsigned)j | code;
+            ++nh;
+        }
+        if (__any(nh == kHitCap)) {
+            drain_hits<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, ri, hits, nh, fx, fy, fz,
+                                           err, orig);
+            nh = 0;
+        }
+    }
+}
+
 // Force on the particle in slot s (position xi,yi,zi, radius ri).
-// hits points at this lane's column of the block's LDS hit list.
-template <int MASK, bool EXACT, int BLOCK = kBlock, class Src, class Tab>
+// hits points at this lane's column of the block's LDS hit list.  With
+// SPLIT > 1, SPLIT consecutive lanes share the particle and lane `sub` walks
+// the neighbour cells sub, sub + SPLIT, ... of the 27 (its partial force;
+// the caller sums the SPLIT partials).
+template <int MASK, bool EXACT, int BLOCK = kBlock, int SPLIT = 1, class Src, class Tab>
 __device__ void gather_force(const Geo &g, const Pot &p, const Src &src,
                              const Tab &tab, int s, double xi, double yi,
                              double zi, double ri, unsigned *hits, double &fx,
                              double &fy, double &fz, DevErr *err,
-                             const Orig &orig)
+                             const Orig &orig, int sub = 0)
 {
     const int S = g.S;
     int cx = cell_1d(xi, g.cell, S);
     int cy = cell_1d(yi, g.cell, S);
     int cz = cell_1d(zi, g.cell, S);
     int nh = 0;
+    if constexpr (SPLIT > 1) {
+        // few particles: more lanes per particle, so more loads in flight
+        for (int q = sub; q < 27; q += SPLIT) {
+            int rx = cx + q / 9 - 1, ry = cy + (q / 3) % 3 - 1, rz = cz + q % 3 - 1;
+            int kx = 0, ky = 0, kz = 0;
+            if (rx < 0) { rx += S; kx = 1; } else if (rx >= S) { rx -= S; kx = 2; }
+            if (ry < 0) { ry += S; ky = 1; } else if (ry >= S) { ry -= S; ky = 2; }
+            if (rz < 0) { rz += S; kz = 1; } else if (rz >= S) { rz -= S; kz = 2; }
+            int b, e;
+            tab.range(g.c0 + rx + S * ry + g.S2 * rz, b, e);
+            cull_cell<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, shifted(xi, kx, g.L),
+                                          shifted(yi, ky, g.L), shifted(zi, kz, g.L), b, e,
+                                          (unsigned)(kx + 3 * ky + 9 * kz) << kIdxBits, hits,
+                                          nh, ri, fx, fy, fz, err, orig);
+        }
+        drain_hits<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, ri, hits, nh, fx, fy, fz,
+                                       err, orig);
+        return;
+    }
     for (int ox = -1; ox <= 1; ++ox) {
         int rx = cx + ox, kx = 0;
         if (rx < 0) { rx += S; kx = 1; } else if (rx >= S) { rx -= S; kx = 2; }

@@ -13,6 +13,10 @@ constexpr double kDenseMinOccupancy = 2.0;
 // below this many particles the thread-per-particle kernels use 64-lane
 // blocks (256-lane blocks would leave CUs idle: 256 CUs x 4 blocks)
 constexpr int kSmallSystem = 256 * 1024;
+#ifndef PSIM_SMALL_SPLIT
+#define PSIM_SMALL_SPLIT 4
+#endif
+constexpr int kSmallSplit = PSIM_SMALL_SPLIT;   // lanes per particle below kSmallSystem
 
 // Step control kept in device memory so that a captured sequence of steps
 // can be replayed: the step kernels read it, the reorder kernel advances it.
@@ -150,14 +154,29 @@ __device__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int
     a.arank[s] = wave_aggregated_rank(a.count, k);
 }
 
-// Thread-per-particle step (sparse systems and S < 4).
-template <int MASK, int BLOCK>
+// Sum over the SPLIT consecutive lanes that share a particle (a fixed
+// butterfly: every lane of the group gets the same bits).
+template <int SPLIT>
+__device__ __forceinline__ double split_sum(double v)
+{
+#pragma unroll
+    for (int d = 1; d < SPLIT; d <<= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Thread-per-particle step (sparse systems and S < 4); with SPLIT > 1,
+// SPLIT lanes per particle share its 27 neighbour cells (small systems,
+// where one lane per particle leaves most SIMDs idle and each lane waits
+// on 27 dependent cell lookups).
+template <int MASK, int BLOCK, int SPLIT = 1>
 __global__ __launch_bounds__(BLOCK) void k_step(Geo g, Pot p, GB gb, double radius,
                                                 StepArgs a)
 {
+    static_assert(SPLIT >= 1 && 64 % SPLIT == 0 && (SPLIT & (SPLIT - 1)) == 0, "lane groups");
     __shared__ unsigned hits[kHitCap * BLOCK];
-    const int s = blockIdx.x * BLOCK + threadIdx.x;
-    if (s >= g.n) return;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    const int s = t / SPLIT, sub = t % SPLIT;
+    if (s >= g.n) return;   // whole lane groups
     const int N = g.n;
     const double xi = a.pos[s], yi = a.pos[N + s], zi = a.pos[2 * N + s];
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -165,11 +184,19 @@ __global__ __launch_bounds__(BLOCK) void k_step(Geo g, Pot p, GB gb, double radi
     StartTab tab{a.cstart};
     Orig orig{a.id, 1, 0};
     if (g.exact_mi)
-        gather_force<MASK, true, BLOCK>(g, p, src, tab, s, xi, yi, zi, radius,
-                                        hits + threadIdx.x, fx, fy, fz, a.err, orig);
+        gather_force<MASK, true, BLOCK, SPLIT>(g, p, src, tab, s, xi, yi, zi, radius,
+                                               hits + threadIdx.x, fx, fy, fz, a.err, orig,
+                                               sub);
     else
-        gather_force<MASK, false, BLOCK>(g, p, src, tab, s, xi, yi, zi, radius,
-                                         hits + threadIdx.x, fx, fy, fz, a.err, orig);
+        gather_force<MASK, false, BLOCK, SPLIT>(g, p, src, tab, s, xi, yi, zi, radius,
+                                                hits + threadIdx.x, fx, fy, fz, a.err, orig,
+                                                sub);
+    if constexpr (SPLIT > 1) {
+        fx = split_sum<SPLIT>(fx);
+        fy = split_sum<SPLIT>(fy);
+        fz = split_sum<SPLIT>(fz);
+        if (sub != 0) return;
+    }
     integrate_one<true>(g, gb, a, s, fx, fy, fz);
 }
 
