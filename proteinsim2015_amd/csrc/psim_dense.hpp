@@ -38,9 +38,16 @@ static_assert(kDenseThreads >= 64 && kDenseThreads % 64 == 0, "whole wave64s per
 template <int T, int HC>
 struct DenseSharedT {
     double sx[kStageCap], sy[kStageCap], sz[kStageCap];
+    union {
+        // the cull's copy: FP32, relative to the home cell's corner
+        struct {
+            float x[kStageCap], y[kStageCap], z[kStageCap];
+        } cf;
+        double red[3][T];   // per-lane partial forces (after the last drain)
+    } u;
     int sslot[kStageCap];
     unsigned short hits[HC * T];
-    double red[3][T];
+    double etab[64];   // copy of kExp2Tab
     int coff[28];
     int cbase[27];
     double shx[27], shy[27], shz[27];
@@ -79,7 +86,7 @@ __device__ void dense_drain(const Geo &g, const Pot &p, const DenseSharedT<T, HC
         const double rinv = rsqrt_nr(r2);
         const double r = r2 * rinv;
         bool nan_ao = false;
-        const double f = pair_fnet<MASK>(p, r2, rinv, r, size, &nan_ao);
+        const double f = pair_fnet<MASK, true>(p, r2, rinv, r, size, &nan_ao, sh.etab);
         if (nan_ao) report_error(err, 7705, id_i, id[sh.sslot[j]]);
         const double fr = f * rinv;
         fx = fma(fr, dx, fx);
@@ -105,11 +112,13 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
     const int home0 = cstart[c];
     const int ni = cstart[c + 1] - home0;
     if (ni == 0) return;   // uniform over the workgroup
+    // origin of the cull's FP32 coordinates: the home cell's corner
+    const double ox = cx * g.cell, oy = cy * g.cell, oz = cz * g.cell;
     // ---- neighbour-cell metadata (lanes 0..26, stencil order)
     int cnt = 0;
     if (lane < 27) {
-        const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
-        int rx = cx + ox, ry = cy + oy, rz = cz + oz;
+        const int dxc = lane / 9 - 1, dyc = (lane / 3) % 3 - 1, dzc = lane % 3 - 1;
+        int rx = cx + dxc, ry = cy + dyc, rz = cz + dzc;
         double sxv = 0.0, syv = 0.0, szv = 0.0;
         if (rx < 0) { rx += S; sxv = -g.L; } else if (rx >= S) { rx -= S; sxv = g.L; }
         if (ry < 0) { ry += S; syv = -g.L; } else if (ry >= S) { ry -= S; syv = g.L; }
@@ -123,6 +132,7 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
         sh.shz[lane] = szv;
     }
     if (lane < 64) {
+        sh.etab[lane] = kExp2Tab[lane];
         int incl = cnt;
 #pragma unroll
         for (int d = 1; d < 32; d <<= 1) {
@@ -138,6 +148,10 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
     const double size = 2.0 * radius;
     const double ov = p.ov_factor * size;
     const double ov2_hi = ov * ov * (1.0 + p.ov_band);
+    // FP32 cull: coordinates below 2 cells in magnitude carry errors below
+    // 2.5e-7 cell each, so r^2 is within 2e-6 cell^2 of the FP64 value;
+    // the threshold is widened by ten times that
+    const float rc2_f = (float)(p.rc2_hi + 2e-5 * g.cell * g.cell);
 
     for (int i0 = 0; i0 < ni; i0 += T) {
         const int nic = min(T, ni - i0);
@@ -147,6 +161,7 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
         const int si = home0 + i0 + gi;       // global slot of this lane's i
         const int hi = hb + i0 + gi;          // staged index of i
         const double xi = pos[si], yi = pos[N + si], zi = pos[2 * N + si];
+        const float xf = (float)(xi - ox), yf = (float)(yi - oy), zf = (float)(zi - oz);
         const int id_i = id[si];
         double fx = 0.0, fy = 0.0, fz = 0.0;
         for (int s0 = 0; s0 < M; s0 += kStageCap) {
@@ -162,9 +177,14 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
                 const double ex = sh.shx[q], ey = sh.shy[q], ez = sh.shz[q];
                 for (int t = t_lo + u; t < t_hi; t += kPer) {
                     const int slot = base + t, k = t - s0;
-                    sh.sx[k] = pos[slot] + ex;
-                    sh.sy[k] = pos[N + slot] + ey;
-                    sh.sz[k] = pos[2 * N + slot] + ez;
+                    const double x = pos[slot] + ex, y = pos[N + slot] + ey,
+                                 z = pos[2 * N + slot] + ez;
+                    sh.sx[k] = x;
+                    sh.sy[k] = y;
+                    sh.sz[k] = z;
+                    sh.u.cf.x[k] = (float)(x - ox);
+                    sh.u.cf.y[k] = (float)(y - oy);
+                    sh.u.cf.z[k] = (float)(z - oz);
                     sh.sslot[k] = slot;
                 }
             }
@@ -177,10 +197,11 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const int jj = j + u * G;
-                        const double dx = sh.sx[jj] - xi, dy = sh.sy[jj] - yi, dz = sh.sz[jj] - zi;
-                        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                        const float dx = sh.u.cf.x[jj] - xf, dy = sh.u.cf.y[jj] - yf,
+                                    dz = sh.u.cf.z[jj] - zf;
+                        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                         sh.hits[nh * T + lane] = (unsigned short)jj;
-                        nh += (r2 < p.rc2_hi) ? 1 : 0;
+                        nh += (r2 < rc2_f) ? 1 : 0;
                     }
                     if (__any(nh > HC - 4)) {
                         dense_drain<MASK, T, HC>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, hi - s0,
@@ -189,26 +210,28 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
                     }
                 }
                 for (; j < mc; j += G) {
-                    const double dx = sh.sx[j] - xi, dy = sh.sy[j] - yi, dz = sh.sz[j] - zi;
-                    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    const float dx = sh.u.cf.x[j] - xf, dy = sh.u.cf.y[j] - yf,
+                                dz = sh.u.cf.z[j] - zf;
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                     sh.hits[nh * T + lane] = (unsigned short)j;
-                    nh += (r2 < p.rc2_hi) ? 1 : 0;
+                    nh += (r2 < rc2_f) ? 1 : 0;
                 }
                 dense_drain<MASK, T, HC>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, hi - s0, lane, nh,
                                   fx, fy, fz, err, id_i, id);
             }
         }
         // ---- sum the G group partials of each home particle
-        sh.red[0][lane] = fx;
-        sh.red[1][lane] = fy;
-        sh.red[2][lane] = fz;
+        __syncthreads();   // the cull's copy is no longer read
+        sh.u.red[0][lane] = fx;
+        sh.u.red[1][lane] = fy;
+        sh.u.red[2][lane] = fz;
         __syncthreads();
         if (lane < nic) {
             double Fx = 0.0, Fy = 0.0, Fz = 0.0;
             for (int q = 0; q < G; ++q) {
-                Fx += sh.red[0][lane + q * nic];
-                Fy += sh.red[1][lane + q * nic];
-                Fz += sh.red[2][lane + q * nic];
+                Fx += sh.u.red[0][lane + q * nic];
+                Fy += sh.u.red[1][lane + q * nic];
+                Fz += sh.u.red[2][lane + q * nic];
             }
             epi(i0 + lane, Fx, Fy, Fz);
         }
