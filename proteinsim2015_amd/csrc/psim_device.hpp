@@ -196,7 +196,8 @@ static __device__ const double kExp2Tab[64] = {
 };
 
 // e^{-a} for 0 <= a < 700 with a 64-entry table (tab = kExp2Tab or a copy
-// of it): a = (64 e + j) ln2/64 + f with |f| <= ln2/128 (two-part ln2/64),
+// of it): a = (64 e + j) ln2/64 + f with |f| <= ln2/128 (ln2/64 in two parts,
+// from the hi/lo split of ln 2 in fdlibm's e_exp.c),
 // e^{-f} by its Taylor series to degree 6 (truncation < 3e-20 relative),
 // times 2^{-j/64} from the table, scaled by 2^{-e}.
 __device__ __forceinline__ double exp_neg_tab(double a, const double *tab)
