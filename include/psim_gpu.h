@@ -101,7 +101,8 @@ typedef struct psim_params {
     int force_mask;         /* PSIM_FORCE_* bits                            */
     int lj_num;             /* ljNum                                        */
     int kernel_path;        /* 0 auto (by density), 1 thread per particle,
-                               2 cell per workgroup (needs S >= 4)          */
+                               2 cell per workgroup (needs S >= 4),
+                               3 brick of cells per workgroup               */
     double cutoff;          /* cutOff (shared by all modules)               */
     double lj_well_depth;   /* wellDepth of the LJ module (epsilon)         */
     double yukawa_strength; /* yukawaStrength                               */
@@ -151,6 +152,12 @@ int psim_gpu_step_device_noise(psim_ctx *ctx, int nsteps, const double *d_noise)
 /* The parameters the context was created with (after psim_gpu_set_force_params). */
 int psim_gpu_get_params(psim_ctx *ctx, psim_params *out);
 
+/* The pair kernel the context runs: *path = 1 thread per particle, 2 cell
+ * per workgroup, 3 brick of cells per workgroup; brick (may be NULL) gets
+ * the brick's home cells along x, y, z (0 unless path 3) and lanes per
+ * particle.  No reference counterpart (the reference has one CPU loop). */
+int psim_gpu_kernel_info(psim_ctx *ctx, int *path, int brick[4]);
+
 /* Forces at the current positions (AoS3, original order); no state change. */
 int psim_gpu_forces(psim_ctx *ctx, double *force_out);
 
@@ -186,7 +193,9 @@ void *psim_gpu_stream(psim_ctx *ctx);
  * together: one launch per kernel stage per step for all replicas, and the
  * step sequence can be captured once in a hipGraph and replayed.  Replicas
  * share the integrator parameters and force modules of `common` (kT, gamma,
- * mass, dt, velFreq, seed, radius, force_mask); geometry and force strengths
+ * mass, dt, velFreq, seed, radius, force_mask; kernel_path 3 starts with
+ * bricks of cells per workgroup instead of the thread-per-particle pair
+ * kernel, with identical results); geometry and force strengths
  * are per replica.  Arrays are AoS3 concatenated in replica order (replica r
  * owns original ids [sum_{q<r} n_q, sum_{q<=r} n_q)).  No reference
  * counterpart: the reference runs one system per process. */
@@ -221,9 +230,12 @@ int psim_ens_set_graph(psim_ens *ens, int steps_per_graph);
 int psim_ens_step(psim_ens *ens, int nsteps);
 int psim_ens_sync(psim_ens *ens);
 int psim_ens_last_error(psim_ens *ens, int *code, int *i, int *j);
-/* measured choice of the pair kernel's workgroup size (64/128/256); the
- * trial steps advance the ensemble */
-int psim_ens_autotune(psim_ens *ens, int trial_steps, int *chosen_block, float *ms_per_step);
+/* measured choice of the pair kernel: *kernel 1 = thread per particle with
+ * workgroup size *tile (64/128/256), 3 = bricks of cells per workgroup with
+ * about *tile home particles per brick (96/192/384); the trial steps advance
+ * the ensemble and the choice changes no result */
+int psim_ens_autotune(psim_ens *ens, int trial_steps, int *kernel, int *tile,
+                      float *ms_per_step);
 
 /* ---- analysis of the current state --------------------------------------- */
 

@@ -73,7 +73,7 @@ EXPORTS = (
     "psim_adj_backward", "psim_init_reference",
     "psim_gpu_get_params", "psim_gpu_get_tracker", "psim_gpu_set_tracker",
     "psim_write_recovery", "psim_read_recovery", "psim_write_xyz", "psim_gpu_save",
-    "psim_gpu_load", "psim_write_sys_config",
+    "psim_gpu_load", "psim_write_sys_config", "psim_gpu_kernel_info",
 )
 
 _vp = C.c_void_p
@@ -111,7 +111,7 @@ _SIGS = {
     "psim_ens_step": ([_vp, C.c_int], C.c_int),
     "psim_ens_sync": ([_vp], C.c_int),
     "psim_ens_last_error": ([_vp, _ip, _ip, _ip], C.c_int),
-    "psim_ens_autotune": ([_vp, C.c_int, _ip, C.POINTER(C.c_float)], C.c_int),
+    "psim_ens_autotune": ([_vp, C.c_int, _ip, _ip, C.POINTER(C.c_float)], C.c_int),
     "psim_gpu_set_force_params": ([_vp, C.POINTER(PsimParams)], C.c_int),
     "psim_gpu_track_start": ([_vp], C.c_int),
     "psim_gpu_observables": ([_vp, C.c_double, C.POINTER(Observables), _ip], C.c_int),
@@ -128,6 +128,7 @@ _SIGS = {
     "psim_init_reference": ([C.c_int, C.c_int, C.c_double, C.c_uint, C.c_int, C.c_double, _dp,
                              _dp, _dp], C.c_int),
     "psim_gpu_get_params": ([_vp, C.POINTER(PsimParams)], C.c_int),
+    "psim_gpu_kernel_info": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "psim_gpu_get_tracker": ([_vp, _dp, _ip], C.c_int),
     "psim_gpu_set_tracker": ([_vp, _dp, _ip], C.c_int),
     "psim_write_recovery": ([C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double],

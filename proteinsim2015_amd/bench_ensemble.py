@@ -55,7 +55,7 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
     # ---- batched: one launch per kernel stage for all replicas, graph replay
     with E.Ensemble(reps, integ) as ens:
         ens.set_state(np.concatenate(pos), np.concatenate(vel))
-        block, _ = ens.autotune(3)
+        kernel, tile, _ = ens.autotune(3)
         ens.set_graph(graph_steps)
         ens.step(warmup)
         ens.sync()
@@ -97,7 +97,8 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
         "per_replica_ms_per_step": (t4 - t3) / baseline_steps * 1e3,
         "per_replica_steps_timed": baseline_steps,
         "speedup": batched / single,
-        "autotuned_block": block,
+        "autotuned_kernel": kernel,
+        "autotuned_tile": tile,
         "graph_steps": graph_steps,
     }
 

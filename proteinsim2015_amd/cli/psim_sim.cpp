@@ -195,7 +195,10 @@ struct System {
         prm.dt = dt;
         prm.vel_freq = cfg.get_int("velFreq", 1000);
         prm.seed = (unsigned long long)(long long)seed;
-        prm.kernel_path = kernel_path == "sparse" ? 1 : kernel_path == "dense" ? 2 : 0;
+        prm.kernel_path = kernel_path == "sparse"  ? 1
+                          : kernel_path == "dense" ? 2
+                          : kernel_path == "brick" ? 3
+                                                   : 0;
         check(psim_gpu_create(&prm, &ctx), "psim_gpu_create");
         std::printf("---Number of Particles: %d\n---Box Size: %d\n---Cell Size: %d\n\n", n, box,
                     cell);
@@ -372,7 +375,7 @@ void usage()
 {
     std::fprintf(stderr,
                  "usage: psim_sim [-c settings.cfg] [-r DIR STAMP [--from-recovery]]\n"
-                 "                [--kernel-path auto|sparse|dense]\n"
+                 "                [--kernel-path auto|sparse|dense|brick]\n"
                  "       psim_sim -a DIR STAMP [--coorhist|-CH] [--clusthist|-CLH] "
                  "[--clustcoor|-CLC]\n");
     std::exit(2);

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <vector>
 
+#include "psim_brick_shape.hpp"
 #include "psim_device.hpp"
 #include "psim_gpu.h"
 
@@ -20,6 +21,9 @@ struct psim_ctx {
     psim::GB gb;
     int n = 0, ncell = 0, nscan = 0;
     bool dense = false;   // cell-per-workgroup pair kernel (else thread per particle)
+    bool brick = false;   // brick-per-workgroup pair kernel (sparse systems)
+    psim::BrickGeo bg{};  // its bricks
+    int brick_g = 1;      // its lanes per particle
     hipStream_t stream = nullptr;
     double *pos = nullptr, *pos0 = nullptr, *frc = nullptr, *mem = nullptr;
     double *npos = nullptr, *npos0 = nullptr, *nfrc = nullptr, *nmem = nullptr;

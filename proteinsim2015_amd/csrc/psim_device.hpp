@@ -360,4 +360,14 @@ __device__ inline double safe_mod0(double v0, double v, double L)
     return (fabs(dx) > (L / 2)) ? ((dx < 0) ? v0 - L : v0 + L) : v0;
 }
 
+// Sum over the SPLIT consecutive lanes that share a particle (a fixed
+// butterfly: every lane of the group gets the same bits).
+template <int SPLIT>
+__device__ __forceinline__ double split_sum(double v)
+{
+#pragma unroll
+    for (int d = 1; d < SPLIT; d <<= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
 }  // namespace psim

@@ -100,6 +100,14 @@ void launch_step_mask(psim_ctx *c, const StepArgs &a)
         const int per = (c->ncell + 7) / 8;
         k_step_dense<MASK><<<8 * per, kDenseThreads, 0, c->stream>>>(
             c->geo, c->pot, c->prm.radius, a.pos, a.id, a.cstart, a.nfrc, a.err, per);
+    } else if (c->brick) {
+        const int per = (c->bg.nbricks + 7) / 8;
+        if (c->brick_g == 1)
+            k_step_brick<MASK, 1><<<8 * per, kBrickThreads, 0, c->stream>>>(
+                c->geo, c->pot, c->gb, c->prm.radius, c->bg, per, a);
+        else
+            k_step_brick<MASK, kSmallSplit><<<8 * per, kBrickThreads, 0, c->stream>>>(
+                c->geo, c->pot, c->gb, c->prm.radius, c->bg, per, a);
     } else if (c->n < kSmallSystem) {
         // few particles: small blocks and kSmallSplit lanes per particle so
         // that every CU gets work
@@ -135,6 +143,11 @@ void launch_forces_mask(psim_ctx *c)
         const int per = (c->ncell + 7) / 8;
         k_forces_dense<MASK><<<8 * per, kDenseThreads, 0, c->stream>>>(
             c->geo, c->pot, c->prm.radius, c->pos, c->id, c->cstart, c->fout, c->err, per);
+    } else if (c->brick) {
+        const int per = (c->bg.nbricks + 7) / 8;
+        k_forces_brick<MASK><<<8 * per, kBrickThreads, 0, c->stream>>>(
+            c->geo, c->pot, c->prm.radius, c->bg, per, c->pos, c->id, c->cstart, c->fout,
+            c->err);
     } else {
         k_forces<MASK><<<blocks_for(c->n, kBlock), kBlock, 0, c->stream>>>(
             c->geo, c->pot, c->prm.radius, c->pos, c->id, c->cstart, c->fout, c->err);
@@ -346,7 +359,7 @@ int psim_gpu_create(const psim_params *params, psim_ctx **out)
     // the 27-cell stencil repeats cells for S < 3 (double counting in the
     // reference, LJPotential.cpp:154-169)
     if (P.cells_per_dim < 3 || P.cells_per_dim > 1290) return PSIM_E_UNSUPPORTED;
-    if (P.kernel_path < 0 || P.kernel_path > 2) return PSIM_E_ARG;
+    if (P.kernel_path < 0 || P.kernel_path > 3) return PSIM_E_ARG;
     if (!(P.box > 0) || !(P.cell_size > 0) || !(P.cutoff > 0) || !(P.dt > 0) ||
         !(P.mass > 0) || P.force_mask <= 0 || P.force_mask > 7)
         return PSIM_E_ARG;
@@ -359,13 +372,19 @@ int psim_gpu_create(const psim_params *params, psim_ctx **out)
     c->ncell = c->geo.ncell;
     c->nscan = (c->ncell + kScanTile - 1) / kScanTile;
     // kernel path: the cell-per-workgroup kernel needs S >= 4 (shifted image
-    // = minimum image) and pays off once cells hold a few particles
-    if (P.kernel_path == 1)
+    // = minimum image) and pays off once cells hold a few particles; below
+    // that, one thread per particle (bricks of cells per workgroup, path 3,
+    // give the same bits but measured slower at every sparse density)
+    const double occupancy = (double)c->n / c->ncell;
+    if (P.kernel_path == 1 || P.kernel_path == 3)
         c->dense = false;
     else if (P.kernel_path == 2)
         c->dense = c->geo.S >= 4;
     else
-        c->dense = c->geo.S >= 4 && (double)c->n / c->ncell >= kDenseMinOccupancy;
+        c->dense = c->geo.S >= 4 && occupancy >= kDenseMinOccupancy;
+    c->brick = P.kernel_path == 3;
+    c->brick_g = c->n < kSmallSystem ? kSmallSplit : 1;
+    c->bg = brick_shape(c->geo.S, occupancy, kBrickTarget * kBrickThreads / c->brick_g);
     const size_t n3 = 3 * (size_t)c->n;
     int rc = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = PSIM_E_HIP;
@@ -405,6 +424,19 @@ int psim_gpu_get_params(psim_ctx *c, psim_params *out)
 {
     if (!c || !out) return PSIM_E_ARG;
     *out = c->prm;
+    return 0;
+}
+
+int psim_gpu_kernel_info(psim_ctx *c, int *path, int brick[4])
+{
+    if (!c || !path) return PSIM_E_ARG;
+    *path = c->dense ? 2 : (c->brick ? 3 : 1);
+    if (brick) {
+        brick[0] = c->brick ? c->bg.bx : 0;
+        brick[1] = c->brick ? c->bg.by : 0;
+        brick[2] = c->brick ? c->bg.bz : 0;
+        brick[3] = c->brick ? c->brick_g : (c->dense ? 0 : (c->n < kSmallSystem ? kSmallSplit : 1));
+    }
     return 0;
 }
 

@@ -13,9 +13,12 @@
 // once in a hipGraph and replayed.  The step counter, time and velocity-step
 // flags live in device memory (StepCtl) so replays advance them.
 //
-// The pair kernel is the thread-per-particle gather (these systems hold
-// ~0.2-1.5 particles per cell); its workgroup size is chosen by a measured
-// autotune over the candidates in kEnsBlocks.
+// These systems hold ~0.1-1.5 particles per cell.  Two pair kernels give the
+// same bits: the thread-per-particle gather (k_ens_step, the default) and
+// bricks of cells per workgroup (k_ens_step_brick: the launch's workgroups
+// are the bricks of every replica, each replica cut by its own density).
+// The autotune times both families over their tiles (kEnsBlocks workgroup
+// sizes, kEnsTargets brick sizes) and keeps the fastest.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,6 +41,7 @@ struct RepGeo {
     Geo g;       // g.n = Ntot (array stride), g.c0 = first cell of the replica
     Pot p;
     double radius;
+    BrickGeo bg;   // the replica's bricks (brick kernel)
 };
 
 // replica owning original id o: last r with p0[r] <= o
@@ -78,6 +82,32 @@ __global__ __launch_bounds__(BLOCK) void k_ens_step(const RepGeo *__restrict__ r
     integrate_one<true>(g, gb, a, s, fx, fy, fz);
 }
 
+// Brick kernel: workgroup `it` of the launch is brick it - item0[r] of
+// replica r (item0 = prefix of the replicas' brick counts).
+template <int MASK>
+__global__ __launch_bounds__(kBrickThreads) void k_ens_step_brick(
+    const RepGeo *__restrict__ reps, const int *__restrict__ item0, int nrep, int nitems,
+    int per_xcd, GB gb, StepArgs a)
+{
+    __shared__ BrickShared sh;
+    const int it = xcd_cell(blockIdx.x, per_xcd);
+    if (it >= nitems) return;
+    const int r = replica_of(item0, nrep, it);
+    const Geo g = reps[r].g;
+    const Pot p = reps[r].p;
+    const double radius = reps[r].radius;
+    const BrickGeo bg = reps[r].bg;
+    const int b = it - item0[r];
+    auto epi = [&](int s, double fx, double fy, double fz) {
+        integrate_one<true>(g, gb, a, s, fx, fy, fz);
+    };
+    if (g.exact_mi)
+        brick_forces<MASK, true, 1>(g, p, radius, a.pos, a.cstart, a.id, a.err, bg, b, sh, epi);
+    else
+        brick_forces<MASK, false, 1>(g, p, radius, a.pos, a.cstart, a.id, a.err, bg, b, sh,
+                                     epi);
+}
+
 // Initial hash of every particle into its replica's cells.
 __global__ void k_ens_hash(const RepGeo *__restrict__ reps, const int *__restrict__ p0,
                            int nrep, int ntot, const double *pos, const int *id, int *key,
@@ -104,6 +134,8 @@ __global__ void k_ctl_advance(StepCtl *c)
 }
 
 constexpr int kEnsBlocks[3] = {64, 128, 256};
+// home particles per brick tried by the autotune (one lane per particle)
+constexpr int kEnsTargets[3] = {96, 192, 384};
 
 }  // namespace
 
@@ -112,7 +144,13 @@ struct psim_ens {
     GB gb;
     int nrep = 0, ntot = 0, ncell = 0, nscan = 0;
     int mask = 0;
-    int block = 256;
+    bool brick = false;  // brick kernel (else thread per particle)
+    int block = 256;     // thread-per-particle workgroup size
+    int target = 192;    // brick kernel: home particles per brick
+    int nitems = 0;      // bricks over all replicas
+    int *item0 = nullptr;
+    std::vector<RepGeo> h_reps;
+    std::vector<int> h_occ_n;
     hipStream_t stream = nullptr;
     RepGeo *reps = nullptr;
     int *p0 = nullptr;
@@ -180,6 +218,12 @@ void launch_ens_pair(psim_ens *e, const StepArgs &a)
 template <int MASK>
 void launch_ens_pair_block(psim_ens *e, const StepArgs &a)
 {
+    if (e->brick) {
+        const int per = (e->nitems + 7) / 8;
+        k_ens_step_brick<MASK><<<8 * per, kBrickThreads, 0, e->stream>>>(
+            e->reps, e->item0, e->nrep, e->nitems, per, e->gb, a);
+        return;
+    }
     switch (e->block) {
     case 64: launch_ens_pair<MASK, 64>(e, a); break;
     case 128: launch_ens_pair<MASK, 128>(e, a); break;
@@ -228,6 +272,28 @@ void drop_graph(psim_ens *e)
     e->graph = nullptr;
 }
 
+// Cut every replica into bricks of about `target` home particles and upload
+// the replica table and the brick prefix.
+int set_bricks(psim_ens *e, int target)
+{
+    std::vector<int> it0(e->nrep + 1);
+    int nitems = 0;
+    for (int r = 0; r < e->nrep; ++r) {
+        RepGeo &rg = e->h_reps[r];
+        const double occ = (double)e->h_occ_n[r] / rg.g.ncell;
+        rg.bg = brick_shape(rg.g.S, occ, (double)target);
+        it0[r] = nitems;
+        nitems += rg.bg.nbricks;
+    }
+    it0[e->nrep] = nitems;
+    ECK(hipMemcpy(e->reps, e->h_reps.data(), sizeof(RepGeo) * e->nrep,
+                  hipMemcpyHostToDevice));
+    ECK(hipMemcpy(e->item0, it0.data(), sizeof(int) * (e->nrep + 1), hipMemcpyHostToDevice));
+    e->target = target;
+    e->nitems = nitems;
+    return 0;
+}
+
 int capture_graph(psim_ens *e)
 {
     drop_graph(e);
@@ -265,6 +331,7 @@ int psim_ens_create(const psim_params *common, const psim_replica *reps, int nre
     e->common = C;
     e->mask = C.force_mask;
     e->nrep = nrep;
+    e->brick = C.kernel_path == 3;
     std::vector<RepGeo> h(nrep);
     long long ntot = 0, ncell = 0;
     e->h_p0.resize(nrep + 1);
@@ -298,6 +365,7 @@ int psim_ens_create(const psim_params *common, const psim_replica *reps, int nre
         h[r].g = g;
         h[r].p = p;
         h[r].radius = C.radius;
+        h[r].bg = BrickGeo{};
         e->h_p0[r] = (int)ntot;
         e->h_c0[r] = (int)ncell;
         e->h_box[r] = R.box;
@@ -331,8 +399,11 @@ int psim_ens_create(const psim_params *common, const psim_replica *reps, int nre
     if (!rc) rc = ealloc(&e->ctl, 1);
     if (!rc) rc = ealloc(&e->reps, (size_t)nrep);
     if (!rc) rc = ealloc(&e->p0, (size_t)nrep + 1);
-    if (!rc && hipMemcpy(e->reps, h.data(), sizeof(RepGeo) * nrep, hipMemcpyHostToDevice) != hipSuccess)
-        rc = PSIM_E_HIP;
+    if (!rc) rc = ealloc(&e->item0, (size_t)nrep + 1);
+    e->h_reps = h;
+    e->h_occ_n.resize(nrep);
+    for (int r = 0; r < nrep; ++r) e->h_occ_n[r] = reps[r].n_particles;
+    if (!rc) rc = set_bricks(e, e->target);
     if (!rc && hipMemcpy(e->p0, e->h_p0.data(), sizeof(int) * (nrep + 1), hipMemcpyHostToDevice) != hipSuccess)
         rc = PSIM_E_HIP;
     if (!rc && hipMemset(e->count, 0, sizeof(int) * e->ncell) != hipSuccess) rc = PSIM_E_HIP;
@@ -352,7 +423,7 @@ void psim_ens_destroy(psim_ens *e)
     drop_graph(e);
     void *bufs[] = {e->pos, e->pos0, e->frc, e->mem, e->npos, e->npos0, e->nfrc, e->nmem,
                     e->vel, e->id, e->nid, e->key, e->arank, e->perm, e->count, e->cstart,
-                    e->bsum, e->err, e->ctl, e->reps, e->p0};
+                    e->bsum, e->err, e->ctl, e->reps, e->p0, e->item0};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -490,20 +561,31 @@ int psim_ens_last_error(psim_ens *e, int *code, int *i, int *j)
     return 0;
 }
 
-// Time every candidate workgroup size for trial_steps steps (after one
-// warm-up step each) and keep the fastest.  The steps taken while tuning
-// advance the ensemble like any other steps.
-int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_block, float *best_ms)
+// Time every candidate (three workgroup sizes of the thread-per-particle
+// kernel, three brick sizes of the brick kernel) for trial_steps steps (after
+// one warm-up step each) and keep the fastest.  The steps taken while tuning
+// advance the ensemble like any other steps; the choice changes no result.
+int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_kernel, int *chosen_tile,
+                      float *best_ms)
 {
     if (!e || trial_steps <= 0 || !e->have_state) return PSIM_E_ARG;
     hipEvent_t t0, t1;
     ECK(hipEventCreate(&t0));
     ECK(hipEventCreate(&t1));
     float best = 0.f;
-    int best_block = e->block;
-    for (int b : kEnsBlocks) {
-        e->block = b;
-        int rc = enqueue_step(e);
+    bool best_brick = e->brick;
+    int best_block = e->brick ? e->target : e->block;
+    for (int ci = 0; ci < 6; ++ci) {
+        const bool brick = ci >= 3;
+        const int b = brick ? kEnsTargets[ci - 3] : kEnsBlocks[ci];
+        int rc = 0;
+        e->brick = brick;
+        if (brick) {
+            if ((rc = set_bricks(e, b))) return rc;
+        } else {
+            e->block = b;
+        }
+        rc = enqueue_step(e);
         if (rc) return rc;
         ECK(hipEventRecord(t0, e->stream));
         for (int k = 0; k < trial_steps; ++k)
@@ -512,16 +594,24 @@ int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_block, float *be
         ECK(hipEventSynchronize(t1));
         float ms = 0.f;
         ECK(hipEventElapsedTime(&ms, t0, t1));
-        if (best_block == 0 || best == 0.f || ms < best) {
+        if (best == 0.f || ms < best) {
             best = ms;
             best_block = b;
+            best_brick = brick;
         }
     }
-    e->block = best_block;
+    e->brick = best_brick;
+    if (e->brick) {
+        int rc = set_bricks(e, best_block);
+        if (rc) return rc;
+    } else {
+        e->block = best_block;
+    }
     drop_graph(e);   // recaptured with the chosen kernel on the next step
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    if (chosen_block) *chosen_block = best_block;
+    if (chosen_kernel) *chosen_kernel = best_brick ? 3 : 1;
+    if (chosen_tile) *chosen_tile = best_block;
     if (best_ms) *best_ms = best / trial_steps;
     return 0;
 }

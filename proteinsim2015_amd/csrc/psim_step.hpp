@@ -2,6 +2,7 @@
 // kernel paths), the G&B integrator epilogue and force-only evaluations.
 #pragma once
 
+#include "psim_brick.hpp"
 #include "psim_dense.hpp"
 #include "psim_pairs.hpp"
 
@@ -17,6 +18,11 @@ constexpr int kSmallSystem = 256 * 1024;
 #define PSIM_SMALL_SPLIT 4
 #endif
 constexpr int kSmallSplit = PSIM_SMALL_SPLIT;   // lanes per particle below kSmallSystem
+// home particles per brick, as a fraction of the lane groups of one workgroup
+#ifndef PSIM_BRICK_TARGET
+#define PSIM_BRICK_TARGET 0.8
+#endif
+constexpr double kBrickTarget = PSIM_BRICK_TARGET;
 
 // Step control kept in device memory so that a captured sequence of steps
 // can be replayed: the step kernels read it, the reorder kernel advances it.
@@ -154,16 +160,6 @@ __device__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int
     a.arank[s] = wave_aggregated_rank(a.count, k);
 }
 
-// Sum over the SPLIT consecutive lanes that share a particle (a fixed
-// butterfly: every lane of the group gets the same bits).
-template <int SPLIT>
-__device__ __forceinline__ double split_sum(double v)
-{
-#pragma unroll
-    for (int d = 1; d < SPLIT; d <<= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-
 // Thread-per-particle step (sparse systems and S < 4); with SPLIT > 1,
 // SPLIT lanes per particle share its 27 neighbour cells (small systems,
 // where one lane per particle leaves most SIMDs idle and each lane waits
@@ -198,6 +194,25 @@ __global__ __launch_bounds__(BLOCK) void k_step(Geo g, Pot p, GB gb, double radi
         if (sub != 0) return;
     }
     integrate_one<true>(g, gb, a, s, fx, fy, fz);
+}
+
+// Brick-per-workgroup step (sparse systems): G lanes per home particle, the
+// G&B epilogue fused as in k_step (same result as k_step<MASK, *, G>).
+template <int MASK, int G>
+__global__ __launch_bounds__(kBrickThreads) void k_step_brick(Geo g, Pot p, GB gb,
+                                                              double radius, BrickGeo bg,
+                                                              int per_xcd, StepArgs a)
+{
+    __shared__ BrickShared sh;
+    const int b = xcd_cell(blockIdx.x, per_xcd);
+    if (b >= bg.nbricks) return;
+    auto epi = [&](int s, double fx, double fy, double fz) {
+        integrate_one<true>(g, gb, a, s, fx, fy, fz);
+    };
+    if (g.exact_mi)
+        brick_forces<MASK, true, G>(g, p, radius, a.pos, a.cstart, a.id, a.err, bg, b, sh, epi);
+    else
+        brick_forces<MASK, false, G>(g, p, radius, a.pos, a.cstart, a.id, a.err, bg, b, sh, epi);
 }
 
 // Cell-per-workgroup forces for the step (dense systems, S >= 4): the total
@@ -261,6 +276,31 @@ __global__ __launch_bounds__(kBlock) void k_forces(Geo g, Pot p, double radius,
     fout[o] = fx;
     fout[N + o] = fy;
     fout[2 * N + o] = fz;
+}
+
+// Forces only, brick per workgroup (same result as k_forces).
+template <int MASK>
+__global__ __launch_bounds__(kBrickThreads) void k_forces_brick(Geo g, Pot p, double radius,
+                                                                BrickGeo bg, int per_xcd,
+                                                                const double *pos,
+                                                                const int *id,
+                                                                const int *cstart,
+                                                                double *fout, DevErr *err)
+{
+    __shared__ BrickShared sh;
+    const int b = xcd_cell(blockIdx.x, per_xcd);
+    if (b >= bg.nbricks) return;
+    const int N = g.n;
+    auto epi = [&](int s, double fx, double fy, double fz) {
+        const int o = id[s];
+        fout[o] = fx;
+        fout[N + o] = fy;
+        fout[2 * N + o] = fz;
+    };
+    if (g.exact_mi)
+        brick_forces<MASK, true, 1>(g, p, radius, pos, cstart, id, err, bg, b, sh, epi);
+    else
+        brick_forces<MASK, false, 1>(g, p, radius, pos, cstart, id, err, bg, b, sh, epi);
 }
 
 template <int MASK>

@@ -261,7 +261,7 @@ def rsa_positions(n: int, box: float, seed: int, dmin: float = 1.0) -> np.ndarra
 class GpuSystem:
     """One system resident in HBM, advanced by the HIP kernels."""
 
-    KERNEL_PATHS = {"auto": 0, "sparse": 1, "dense": 2}
+    KERNEL_PATHS = {"auto": 0, "sparse": 1, "dense": 2, "brick": 3}
 
     def __init__(self, geometry: SystemGeometry, forces, integrator: BrownianIntegrator,
                  noise_seed: int | None = None, kernel_path: str = "auto"):
@@ -379,6 +379,14 @@ class GpuSystem:
         check(self.lib.psim_gpu_step_device_noise(self._ctx, int(nsteps),
                                                   C.c_void_p(device_ptr)),
               "psim_gpu_step_device_noise")
+
+    def kernel_info(self) -> dict:
+        """The pair kernel this context runs ("sparse", "dense" or "brick"),
+        the brick's home cells and the lanes per particle."""
+        path, b = C.c_int(), (C.c_int * 4)()
+        check(self.lib.psim_gpu_kernel_info(self._ctx, C.byref(path), b), "psim_gpu_kernel_info")
+        return {"path": {1: "sparse", 2: "dense", 3: "brick"}[path.value],
+                "brick": tuple(b[:3]), "lanes_per_particle": b[3]}
 
     def forces_now(self):
         """(status, forces AoS3 by original index) at the current positions."""
@@ -577,7 +585,8 @@ class Ensemble:
     together on the GPU (one launch per kernel stage for all replicas,
     optionally replayed from a captured hipGraph)."""
 
-    def __init__(self, replicas, integrator: BrownianIntegrator, radius: float = 0.5):
+    def __init__(self, replicas, integrator: BrownianIntegrator, radius: float = 0.5,
+                 kernel_path: str = "sparse"):
         self.lib = _lib.load()
         if not replicas:
             raise ValueError("empty ensemble")
@@ -594,6 +603,9 @@ class Ensemble:
                                                            integrator.gamma, integrator.dt)
         common.vel_freq = integrator.vel_freq
         common.seed = integrator.seed
+        if kernel_path not in ("brick", "sparse"):
+            raise ValueError("ensemble kernel_path is 'sparse' or 'brick'")
+        common.kernel_path = 3 if kernel_path == "brick" else 1
         arr = (_lib.PsimReplica * len(replicas))()
         for k, (rep, fs) in enumerate(zip(replicas, sets)):
             tmp = PsimParams()
@@ -662,7 +674,12 @@ class Ensemble:
         return code.value, i.value, j.value
 
     def autotune(self, trial_steps: int = 5):
-        b, ms = C.c_int(), C.c_float()
-        check(self.lib.psim_ens_autotune(self._ctx, int(trial_steps), C.byref(b), C.byref(ms)),
-              "psim_ens_autotune")
-        return b.value, ms.value
+        """Time the pair-kernel candidates (thread per particle with workgroup
+        sizes 64/128/256, bricks of about 96/192/384 home particles) and keep
+        the fastest; returns (kernel, tile, ms per step), kernel "sparse" or
+        "brick".  The trial steps advance the ensemble; results do not depend
+        on the choice."""
+        k, t, ms = C.c_int(), C.c_int(), C.c_float()
+        check(self.lib.psim_ens_autotune(self._ctx, int(trial_steps), C.byref(k), C.byref(t),
+                                         C.byref(ms)), "psim_ens_autotune")
+        return ("brick" if k.value == 3 else "sparse"), t.value, ms.value

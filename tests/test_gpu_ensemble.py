@@ -74,8 +74,11 @@ def test_graph_replay_and_autotune_are_bit_identical():
                 ens.set_graph(5)
                 ens.step(12)
             elif mode == "tuned":
-                block, ms = ens.autotune(3)   # 3 candidates x (1 + 3) steps = 12 steps
-                assert block in (64, 128, 256) and ms > 0
+                # 6 candidates x (1 + 1) steps = 12 steps
+                kernel, tile, ms = ens.autotune(1)
+                assert (kernel, tile) in [("sparse", 64), ("sparse", 128), ("sparse", 256),
+                                          ("brick", 96), ("brick", 192), ("brick", 384)]
+                assert ms > 0
             else:
                 ens.step(12)
             ens.sync()

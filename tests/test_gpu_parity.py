@@ -28,7 +28,7 @@ def modules_for(plugin, cfg):
     return cls.from_config(ps.Config(cfg))
 
 
-PATHS = ["sparse", "dense"]
+PATHS = ["sparse", "dense", "brick"]
 
 
 def system_for(plugin, cfg, n, box, cell, scale, integ=None, seed=NOISE_SEED, path="auto"):
