@@ -5,10 +5,10 @@
 // (x outer, z inner; LJPotential.cpp:154-177), with the periodic image shift
 // of each neighbour cell applied once while staging.  The n_i home particles
 // are spread over G = kDenseThreads / n_i lane groups; lane (i, g) culls
-// candidates g, g+G, ... of the staged list, keeps survivors in a per-lane
-// LDS hit list and drains it (pair forces) whenever any lane of its wave
-// fills it.  Group partials are summed through LDS, then lanes 0..n_i-1 run
-// the integrator epilogue for the home particles.
+// candidates g, g+G, ... of the staged list in batches of up to 32, one bit
+// per candidate in a register mask, then evaluates the pair forces of the
+// batch's survivors in candidate order.  Group partials are summed through
+// LDS, then lanes 0..n_i-1 hand the home particles' forces to the epilogue.
 //
 // Neighbour sets are bit-exact with the reference: fast r^2 on the shifted
 // image with a guard band, and util::pbcDist's exact arithmetic on the raw
@@ -27,15 +27,11 @@ namespace psim {
 #ifndef PSIM_STAGE_CAP
 #define PSIM_STAGE_CAP 512
 #endif
-#ifndef PSIM_DENSE_HIT_CAP
-#define PSIM_DENSE_HIT_CAP 16
-#endif
 constexpr int kDenseThreads = PSIM_DENSE_THREADS;  // lanes per home cell (4 wave64s)
 constexpr int kStageCap = PSIM_STAGE_CAP;          // staged candidates per chunk
-constexpr int kDenseHitCap = PSIM_DENSE_HIT_CAP;   // per-lane hit-list entries
 static_assert(kDenseThreads >= 64 && kDenseThreads % 64 == 0, "whole wave64s per cell");
 
-template <int T, int HC>
+template <int T>
 struct DenseSharedT {
     double sx[kStageCap], sy[kStageCap], sz[kStageCap];
     union {
@@ -46,13 +42,12 @@ struct DenseSharedT {
         double red[3][T];   // per-lane partial forces (after the last drain)
     } u;
     int sslot[kStageCap];
-    unsigned short hits[HC * T];
     double etab[64];   // copy of kExp2Tab
     int coff[28];
     int cbase[27];
     double shx[27], shy[27], shz[27];
 };
-using DenseShared = DenseSharedT<kDenseThreads, kDenseHitCap>;
+using DenseShared = DenseSharedT<kDenseThreads>;
 
 // Map a workgroup index to a cell so that each XCD (workgroups are handed
 // out round-robin over 8 XCDs) walks one contiguous slab of cells and its
@@ -62,48 +57,66 @@ __device__ inline int xcd_cell(int b, int per_xcd)
     return (b & 7) * per_xcd + (b >> 3);
 }
 
-template <int MASK, int T, int HC>
-__device__ void dense_drain(const Geo &g, const Pot &p, const DenseSharedT<T, HC> &sh,
-                            const double *__restrict__ pos, double xi, double yi,
-                            double zi, double size, double ov2_hi, int self, int lane,
-                            int nh, double &fx,
-                            double &fy, double &fz, DevErr *err, int id_i,
-                            const int *__restrict__ id)
+// Pair force of staged candidate j on the lane's home particle (FP64; the
+// exact pbcDist recheck for survivors inside the guard band).
+template <int MASK, int T, int LJN>
+__device__ __forceinline__ void dense_pair(const Geo &g, const Pot &p,
+                                           const DenseSharedT<T> &sh,
+                                           const double *__restrict__ pos, double xi, double yi,
+                                           double zi, double size, double ov2_hi, int j,
+                                           double &fx, double &fy, double &fz, DevErr *err,
+                                           int id_i, const int *__restrict__ id)
 {
     const int N = g.n;
-    for (int k = 0; k < nh; ++k) {
-        const int j = sh.hits[k * T + lane];
+    const double dx = sh.sx[j] - xi, dy = sh.sy[j] - yi, dz = sh.sz[j] - zi;
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    if (r2 >= p.rc2_lo || r2 < ov2_hi) {
+        const int sj = sh.sslot[j];
+        const double r2x = pbc_dist_ref(xi, yi, zi, pos[sj], pos[N + sj], pos[2 * N + sj],
+                                        g.L, g.half_pbc);
+        if (r2 >= p.rc2_lo && !(r2x < p.rc2)) return;
+        if (sqrt(r2x) < p.ov_factor * size) report_error(err, 7704, id_i, id[sj]);
+    }
+    const double rinv = rsqrt_nr(r2);
+    const double r = r2 * rinv;
+    bool nan_ao = false;
+    const double f = pair_fnet<MASK, true, LJN>(p, r2, rinv, r, size, &nan_ao, sh.etab);
+    if (nan_ao) report_error(err, 7705, id_i, id[sh.sslot[j]]);
+    const double fr = f * rinv;
+    fx = fma(fr, dx, fx);
+    fy = fma(fr, dy, fy);
+    fz = fma(fr, dz, fz);
+}
+
+// Survivors of one batch of n <= 32 candidates j0, j0 + G, ...: bit
+// (n - 1 - k) of m is set when candidate j0 + k G passed the cull, so taking
+// the highest bit first keeps the candidate order.
+template <int MASK, int T, int LJN>
+__device__ void dense_drain_mask(const Geo &g, const Pot &p, const DenseSharedT<T> &sh,
+                                 const double *__restrict__ pos, double xi, double yi,
+                                 double zi, double size, double ov2_hi, int self, unsigned m,
+                                 int n, int j0, int G, double &fx, double &fy, double &fz,
+                                 DevErr *err, int id_i, const int *__restrict__ id)
+{
+    while (m) {
+        const int b = 31 - __clz(m);
+        m ^= 1u << b;
+        const int j = j0 + (n - 1 - b) * G;
         if (j == self) continue;
-        const double dx = sh.sx[j] - xi, dy = sh.sy[j] - yi, dz = sh.sz[j] - zi;
-        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        if (r2 >= p.rc2_lo || r2 < ov2_hi) {
-            const int sj = sh.sslot[j];
-            const double r2x = pbc_dist_ref(xi, yi, zi, pos[sj], pos[N + sj], pos[2 * N + sj],
-                                            g.L, g.half_pbc);
-            if (r2 >= p.rc2_lo && !(r2x < p.rc2)) continue;
-            if (sqrt(r2x) < p.ov_factor * size) report_error(err, 7704, id_i, id[sj]);
-        }
-        const double rinv = rsqrt_nr(r2);
-        const double r = r2 * rinv;
-        bool nan_ao = false;
-        const double f = pair_fnet<MASK, true>(p, r2, rinv, r, size, &nan_ao, sh.etab);
-        if (nan_ao) report_error(err, 7705, id_i, id[sh.sslot[j]]);
-        const double fr = f * rinv;
-        fx = fma(fr, dx, fx);
-        fy = fma(fr, dy, fy);
-        fz = fma(fr, dz, fz);
+        dense_pair<MASK, T, LJN>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, j, fx, fy, fz, err,
+                                 id_i, id);
     }
 }
 
 // Forces on the home particles of cell c.  On return, lanes 0..ni-1 hold the
 // total force on home particle (i0 + lane) for each i-chunk via the callback
 // epi(i, fx, fy, fz) (called by those lanes only).
-template <int MASK, int T, int HC, class Epi>
+template <int MASK, int T, int LJN, class Epi>
 __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
                                   const double *__restrict__ pos,
                                   const int *__restrict__ cstart,
                                   const int *__restrict__ id, DevErr *err, int c,
-                                  DenseSharedT<T, HC> &sh, Epi epi)
+                                  DenseSharedT<T> &sh, Epi epi)
 {
     const int N = g.n, S = g.S;
     const int lane = threadIdx.x;
@@ -191,33 +204,31 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
             __syncthreads();
             if (act) {
                 // self is accepted by the cull (r^2 = 0) and skipped in the drain
-                int nh = 0;
                 int j = gg;
-                for (; j + 3 * G < mc; j += 4 * G) {
+                while (j < mc) {
+                    const int j0 = j;
+                    unsigned m = 0;
+                    int n = 0;
+                    for (; n <= 28 && j + 3 * G < mc; n += 4, j += 4 * G) {
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int jj = j + u * G;
-                        const float dx = sh.u.cf.x[jj] - xf, dy = sh.u.cf.y[jj] - yf,
-                                    dz = sh.u.cf.z[jj] - zf;
+                        for (int u = 0; u < 4; ++u) {
+                            const int jj = j + u * G;
+                            const float dx = sh.u.cf.x[jj] - xf, dy = sh.u.cf.y[jj] - yf,
+                                        dz = sh.u.cf.z[jj] - zf;
+                            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                            m = (m << 1) | (r2 < rc2_f ? 1u : 0u);
+                        }
+                    }
+                    for (; n < 32 && j < mc; ++n, j += G) {
+                        const float dx = sh.u.cf.x[j] - xf, dy = sh.u.cf.y[j] - yf,
+                                    dz = sh.u.cf.z[j] - zf;
                         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        sh.hits[nh * T + lane] = (unsigned short)jj;
-                        nh += (r2 < rc2_f) ? 1 : 0;
+                        m = (m << 1) | (r2 < rc2_f ? 1u : 0u);
                     }
-                    if (__any(nh > HC - 4)) {
-                        dense_drain<MASK, T, HC>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, hi - s0,
-                                          lane, nh, fx, fy, fz, err, id_i, id);
-                        nh = 0;
-                    }
+                    dense_drain_mask<MASK, T, LJN>(g, p, sh, pos, xi, yi, zi, size, ov2_hi,
+                                                   hi - s0, m, n, j0, G, fx, fy, fz, err, id_i,
+                                                   id);
                 }
-                for (; j < mc; j += G) {
-                    const float dx = sh.u.cf.x[j] - xf, dy = sh.u.cf.y[j] - yf,
-                                dz = sh.u.cf.z[j] - zf;
-                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    sh.hits[nh * T + lane] = (unsigned short)j;
-                    nh += (r2 < rc2_f) ? 1 : 0;
-                }
-                dense_drain<MASK, T, HC>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, hi - s0, lane, nh,
-                                  fx, fy, fz, err, id_i, id);
             }
         }
         // ---- sum the G group partials of each home particle

@@ -221,10 +221,12 @@ __device__ __forceinline__ double exp_neg_tab(double a, const double *tab)
 }
 
 // (size/r)^n for the LJ module, n = ljNum; the common exponents unrolled.
+// LJN > 0 fixes n at compile time (same operations, no branch on n).
+template <int LJN = 0>
 __device__ __forceinline__ double lj_power(double sr, int n)
 {
     const double x = sr * sr;
-    switch (n) {
+    switch (LJN > 0 ? LJN : n) {
     case 6: return x * x * x;
     case 12: { double x2 = x * x; double x4 = x2 * x2; return x4 * x2; }
     case 18: { double x2 = x * x; double x4 = x2 * x2; double x8 = x4 * x4; return x8 * x; }
@@ -236,9 +238,9 @@ __device__ __forceinline__ double lj_power(double sr, int n)
 // Pair force magnitude (positive = attractive) summed over the enabled
 // modules.  r2, rinv = 1/sqrt(r2), r = r2*rinv; size = r_i + r_j.  TAB: the
 // Yukawa exponential through the 2^(-j/64) table at etab (kernels that keep
-// a copy in LDS).
+// a copy in LDS).  LJN: ljNum fixed at compile time (0: p.lj_num).
 // Sets *nan_ao when the AO term is NaN (AOPotential.cpp:105-109).
-template <int MASK, bool TAB = false>
+template <int MASK, bool TAB = false, int LJN = 0>
 __device__ __forceinline__ double pair_fnet(const Pot &p, double r2, double rinv,
                                             double r, double size, bool *nan_ao,
                                             const double *etab = nullptr)
@@ -250,7 +252,7 @@ __device__ __forceinline__ double pair_fnet(const Pot &p, double r2, double rinv
     }
     if (MASK & F_LJ) {
         // LJPotential.cpp:97-109
-        const double lj = lj_power(size * rinv, p.lj_num);
+        const double lj = lj_power<LJN>(size * rinv, p.lj_num);
         const double attract = fma(2.0, lj, -1.0) * (p.lj_four_n * rinv * lj);
         double repel = 0.0;
         if (p.yuk_a != 0.0) {

@@ -22,6 +22,8 @@
 //               sorted arrays.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -93,13 +95,31 @@ int launch_build(psim_ctx *c)
     return 0;
 }
 
+// The common LJ exponents (ljNum 6, 12, 18) get kernels with the exponent
+// fixed at compile time; any other value takes the general kernel.
+template <int MASK, class Launch>
+void with_lj_exponent(int lj_num, Launch launch)
+{
+    if constexpr ((MASK & F_LJ) != 0) {
+        switch (lj_num) {
+        case 18: launch(std::integral_constant<int, 18>{}); return;
+        case 12: launch(std::integral_constant<int, 12>{}); return;
+        case 6: launch(std::integral_constant<int, 6>{}); return;
+        default: break;
+        }
+    }
+    launch(std::integral_constant<int, 0>{});
+}
+
 template <int MASK>
 void launch_step_mask(psim_ctx *c, const StepArgs &a)
 {
     if (c->dense) {
         const int per = (c->ncell + 7) / 8;
-        k_step_dense<MASK><<<8 * per, kDenseThreads, 0, c->stream>>>(
-            c->geo, c->pot, c->prm.radius, a.pos, a.id, a.cstart, a.nfrc, a.err, per);
+        with_lj_exponent<MASK>(c->pot.lj_num, [&](auto ljn) {
+            k_step_dense<MASK, decltype(ljn)::value><<<8 * per, kDenseThreads, 0, c->stream>>>(
+                c->geo, c->pot, c->prm.radius, a.pos, a.id, a.cstart, a.nfrc, a.err, per);
+        });
     } else if (c->brick) {
         const int per = (c->bg.nbricks + 7) / 8;
         if (c->brick_g == 1)
@@ -141,8 +161,12 @@ void launch_forces_mask(psim_ctx *c)
 {
     if (c->dense) {
         const int per = (c->ncell + 7) / 8;
-        k_forces_dense<MASK><<<8 * per, kDenseThreads, 0, c->stream>>>(
-            c->geo, c->pot, c->prm.radius, c->pos, c->id, c->cstart, c->fout, c->err, per);
+        with_lj_exponent<MASK>(c->pot.lj_num, [&](auto ljn) {
+            k_forces_dense<MASK, decltype(ljn)::value>
+                <<<8 * per, kDenseThreads, 0, c->stream>>>(c->geo, c->pot, c->prm.radius, c->pos,
+                                                           c->id, c->cstart, c->fout, c->err,
+                                                           per);
+        });
     } else if (c->brick) {
         const int per = (c->bg.nbricks + 7) / 8;
         k_forces_brick<MASK><<<8 * per, kBrickThreads, 0, c->stream>>>(

@@ -219,7 +219,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_step_brick(Geo g, Pot p, GB g
 // force on each particle goes to nfrc[slot], where k_integrate picks it up.
 // (Fusing the G&B epilogue here would run it on n_i of the workgroup's lanes,
 // latency-bound; the separate kernel runs it at full occupancy.)
-template <int MASK>
+template <int MASK, int LJN>
 __global__ __launch_bounds__(kDenseThreads) void k_step_dense(Geo g, Pot p, double radius,
                                                               const double *pos,
                                                               const int *id,
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_step_dense(Geo g, Pot p, doub
     if (c >= g.ncell) return;
     const int home0 = cstart[c];
     const int N = g.n;
-    dense_cell_forces<MASK, kDenseThreads, kDenseHitCap>(g, p, radius, pos, cstart, id, err, c, sh,
+    dense_cell_forces<MASK, kDenseThreads, LJN>(g, p, radius, pos, cstart, id, err, c, sh,
                             [=](int i, double Fx, double Fy, double Fz) {
                                 const int s = home0 + i;
                                 nfrc[s] = Fx;
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_forces_brick(Geo g, Pot p, do
         brick_forces<MASK, false, 1>(g, p, radius, pos, cstart, id, err, bg, b, sh, epi);
 }
 
-template <int MASK>
+template <int MASK, int LJN>
 __global__ __launch_bounds__(kDenseThreads) void k_forces_dense(Geo g, Pot p, double radius,
                                                                 const double *pos,
                                                                 const int *id,
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_forces_dense(Geo g, Pot p, do
     if (c >= g.ncell) return;
     const int home0 = cstart[c];
     const int N = g.n;
-    dense_cell_forces<MASK, kDenseThreads, kDenseHitCap>(g, p, radius, pos, cstart, id, err, c, sh,
+    dense_cell_forces<MASK, kDenseThreads, LJN>(g, p, radius, pos, cstart, id, err, c, sh,
                             [=](int i, double Fx, double Fy, double Fz) {
                                 const int o = id[home0 + i];
                                 fout[o] = Fx;
