@@ -9,9 +9,9 @@
 // staged-offset table, and copies the halo's positions into LDS in halo
 // order (x fastest), each load independent of the others.  Then G
 // consecutive lanes take one home particle and run the thread-per-particle
-// gather against LDS: lane gg walks the particle's neighbour cells gg,
-// gg + G, ... in the reference's stencil order (x outer, z inner;
-// LJPotential.cpp:154-177), candidates within a cell in slot order, with
+// gather against LDS: lane gg walks the particle's x-runs gg, gg + G, ...
+// (the 27 neighbour cells of LJPotential.cpp:154-177 in gather_force's
+// order: (dy, dz) y outer, then dx = -1, 0, +1), candidates in slot order, with
 // k_step's arithmetic (shifted home position per neighbour cell, guard-banded
 // r^2, util::pbcDist's recheck, pair_fnet), survivors through a per-lane LDS
 // hit list, and the G partials summed by the same xor butterfly.  A brick
@@ -238,39 +238,50 @@ __device__ void brick_forces(const Geo &g, const Pot &p, double radius,
             }
             if (act) {
                 int nh = 0;
-                for (int q = gg; q < 27; q += G) {
-                    const int du = q / 9 - 1, dv = (q / 3) % 3 - 1, dw = q % 3 - 1;
-                    const int rx = x0 + u - 1 + du, ry = y0 + v - 1 + dv, rz = z0 + w - 1 + dw;
-                    const int kx = rx < 0 ? 1 : (rx >= S ? 2 : 0);
+                const int cx = x0 + u - 1;
+                const bool lw = cx == 0, rw = cx == S - 1;
+                for (int q = gg; q < 9; q += G) {
+                    const int dv = q / 3 - 1, dw = q % 3 - 1;
+                    const int ry = y0 + v - 1 + dv, rz = z0 + w - 1 + dw;
                     const int ky = ry < 0 ? 1 : (ry >= S ? 2 : 0);
                     const int kz = rz < 0 ? 1 : (rz >= S ? 2 : 0);
-                    const double xs = shifted(xi, kx, g.L), ys = shifted(yi, ky, g.L),
-                                 zs = shifted(zi, kz, g.L);
-                    const unsigned code = (unsigned)(kx + 3 * ky + 9 * kz) << 10;
-                    const int k = (u + du) + HX * ((v + dv) + HY * (w + dw));
-                    const int jb = max(sh.hoff[k], s0) - s0;
-                    const int je = min(sh.hoff[k + 1], s0 + mc) - s0;
-                    for (int j = jb; j < je; ++j) {
-                        bool hit;
-                        if (!EXACT) {
-                            const double dx = sh.sx[j] - xs;
-                            const double dy = sh.sy[j] - ys;
-                            const double dz = sh.sz[j] - zs;
-                            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                            hit = (r2 < p.rc2_hi) && (sh.sslot[j] != slot);
-                        } else {
-                            const double r2 = pbc_dist_ref(xi, yi, zi, sh.sx[j], sh.sy[j],
-                                                           sh.sz[j], g.L, g.half_pbc);
-                            hit = (r2 < p.rc2) && (sh.sslot[j] != slot);
-                        }
-                        if (hit) {
-                            sh.hits[nh * T + lane] = (unsigned short)((unsigned)j | code);
-                            ++nh;
-                        }
-                        if (__any(nh == kBrickHitCap)) {
-                            brick_drain<MASK, EXACT>(g, p, sh, id, xi, yi, zi, size, slot, nh,
-                                                     fx, fy, fz, err);
-                            nh = 0;
+                    const double ys = shifted(yi, ky, g.L), zs = shifted(zi, kz, g.L);
+                    // halo cells u-1, u, u+1 of this row are consecutive in the
+                    // stage; split where the row wraps, as x_run does
+                    const int krow = HX * ((v + dv) + HY * (w + dw));
+                    const int a_hi = lw ? u - 1 : (rw ? u : u + 1);
+#pragma unroll
+                    for (int sg = 0; sg < 2; ++sg) {
+                        if (sg == 1 && !(lw || rw)) break;
+                        const int c_lo = sg ? a_hi + 1 : u - 1;
+                        const int c_hi = sg ? u + 1 : a_hi;
+                        const int kx = sg ? (lw ? 0 : 2) : (lw ? 1 : 0);
+                        const double xs = shifted(xi, kx, g.L);
+                        const unsigned code = (unsigned)(kx + 3 * ky + 9 * kz) << 10;
+                        const int jb = max(sh.hoff[krow + c_lo], s0) - s0;
+                        const int je = min(sh.hoff[krow + c_hi + 1], s0 + mc) - s0;
+                        for (int j = jb; j < je; ++j) {
+                            bool hit;
+                            if (!EXACT) {
+                                const double dx = sh.sx[j] - xs;
+                                const double dy = sh.sy[j] - ys;
+                                const double dz = sh.sz[j] - zs;
+                                const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                                hit = (r2 < p.rc2_hi) && (sh.sslot[j] != slot);
+                            } else {
+                                const double r2 = pbc_dist_ref(xi, yi, zi, sh.sx[j], sh.sy[j],
+                                                               sh.sz[j], g.L, g.half_pbc);
+                                hit = (r2 < p.rc2) && (sh.sslot[j] != slot);
+                            }
+                            if (hit) {
+                                sh.hits[nh * T + lane] = (unsigned short)((unsigned)j | code);
+                                ++nh;
+                            }
+                            if (__any(nh == kBrickHitCap)) {
+                                brick_drain<MASK, EXACT>(g, p, sh, id, xi, yi, zi, size, slot,
+                                                         nh, fx, fy, fz, err);
+                                nh = 0;
+                            }
                         }
                     }
                 }

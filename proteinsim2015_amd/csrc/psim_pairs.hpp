@@ -1,9 +1,12 @@
 // psim_pairs.hpp -- the 27-cell pair gather shared by every force path.
 //
-// One lane per target particle i.  The lane walks the 27 neighbour cells in
-// the reference's order (x outer, z inner; LJPotential.cpp:154-177) and culls
-// candidates with a cheap FP64 distance on the periodic image of the
-// neighbour cell (shift applied once per cell).  Survivors go to a per-lane
+// One lane per target particle i.  The lane walks the 27 neighbour cells of
+// the reference's stencil (LJPotential.cpp:154-177) as 9 runs along x: for
+// each (dy, dz), y outer and z inner, the cells x-1, x, x+1 of that row hold
+// consecutive slots, so the run is one slot range (two where the row wraps
+// the periodic boundary, each with its own image shift), candidates in slot
+// order.  It culls them with a cheap FP64 distance on the periodic image of
+// the neighbour cell (shift applied once per segment).  Survivors go to a per-lane
 // hit list in LDS; the list is drained (pair forces evaluated) whenever any
 // lane of the wave fills it, so the expensive pair arithmetic runs with
 // (nearly) all lanes busy instead of under a sparse divergent mask.
@@ -48,6 +51,12 @@ struct StartTab {
         b = cs[h];
         e = cs[h + 1];
     }
+    // slots of the consecutive cells h_lo..h_hi of one row
+    __device__ void row_range(int h_lo, int h_hi, int &b, int &e) const
+    {
+        b = cs[h_lo];
+        e = cs[h_hi + 1];
+    }
 };
 
 // The reference's cellStartEnd: (start, end) pairs, -1 for empty
@@ -63,7 +72,43 @@ struct PairTab {
             e = 0;
         }
     }
+    // cells are stored in cell order, so consecutive non-empty cells are
+    // adjacent: the row's slots run from the first start to the last end
+    __device__ void row_range(int h_lo, int h_hi, int &b, int &e) const
+    {
+        b = 0;
+        e = 0;
+        for (int h = h_lo; h <= h_hi; ++h) {
+            int hb, he;
+            range(h, hb, he);
+            if (he > hb) {
+                if (e == 0) b = hb;
+                e = he;
+            }
+        }
+    }
 };
+
+// The neighbour cells x-1, x, x+1 (cell x = cx) of row `row` as at most two
+// slot segments in stencil order dx = -1, 0, +1: a wrapped end cell is a
+// segment of its own with image code 1 (dx = -1 at cx = 0: image +L) or 2
+// (dx = +1 at cx = S-1: image -L).  Segment 1 is empty unless the row wraps.
+struct XRun {
+    int b0, e0, k0, b1, e1, k1;
+};
+
+template <class Tab>
+__device__ __forceinline__ XRun x_run(const Tab &tab, int row, int cx, int S)
+{
+    const bool lw = cx == 0, rw = cx == S - 1;
+    XRun r;
+    tab.row_range(row + (lw ? S - 1 : cx - 1), row + (lw || rw ? S - 1 : cx + 1), r.b0, r.e0);
+    r.k0 = lw ? 1 : 0;
+    tab.row_range(row, row + (lw ? 1 : 0), r.b1, r.e1);
+    if (!(lw || rw)) r.e1 = r.b1;
+    r.k1 = lw ? 0 : 2;
+    return r;
+}
 
 // Original particle index of a slot (for error reports).
 struct Orig {
@@ -168,8 +213,8 @@ __device__ __forceinline__ void cull_cell(const Geo &g, const Pot &p, const Src 
 // Force on the particle in slot s (position xi,yi,zi, radius ri).
 // hits points at this lane's column of the block's LDS hit list.  With
 // SPLIT > 1, SPLIT consecutive lanes share the particle and lane `sub` walks
-// the neighbour cells sub, sub + SPLIT, ... of the 27 (its partial force;
-// the caller sums the SPLIT partials).
+// the x-runs sub, sub + SPLIT, ... of the 9 (its partial force; the caller
+// sums the SPLIT partials).
 template <int MASK, bool EXACT, int BLOCK = kBlock, int SPLIT = 1, class Src, class Tab>
 __device__ void gather_force(const Geo &g, const Pot &p, const Src &src,
                              const Tab &tab, int s, double xi, double yi,
@@ -178,73 +223,29 @@ __device__ void gather_force(const Geo &g, const Pot &p, const Src &src,
                              const Orig &orig, int sub = 0)
 {
     const int S = g.S;
-    int cx = cell_1d(xi, g.cell, S);
-    int cy = cell_1d(yi, g.cell, S);
-    int cz = cell_1d(zi, g.cell, S);
+    const int cx = cell_1d(xi, g.cell, S);
+    const int cy = cell_1d(yi, g.cell, S);
+    const int cz = cell_1d(zi, g.cell, S);
     int nh = 0;
-    if constexpr (SPLIT > 1) {
-        // few particles: more lanes per particle, so more loads in flight
-        for (int q = sub; q < 27; q += SPLIT) {
-            int rx = cx + q / 9 - 1, ry = cy + (q / 3) % 3 - 1, rz = cz + q % 3 - 1;
-            int kx = 0, ky = 0, kz = 0;
-            if (rx < 0) { rx += S; kx = 1; } else if (rx >= S) { rx -= S; kx = 2; }
-            if (ry < 0) { ry += S; ky = 1; } else if (ry >= S) { ry -= S; ky = 2; }
-            if (rz < 0) { rz += S; kz = 1; } else if (rz >= S) { rz -= S; kz = 2; }
-            int b, e;
-            tab.range(g.c0 + rx + S * ry + g.S2 * rz, b, e);
-            cull_cell<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, shifted(xi, kx, g.L),
-                                          shifted(yi, ky, g.L), shifted(zi, kz, g.L), b, e,
+    // SPLIT > 1 (few particles): lane `sub` takes runs sub, sub + SPLIT, ...
+    for (int q = sub; q < 9; q += SPLIT) {
+        int ry = cy + q / 3 - 1, rz = cz + q % 3 - 1;
+        int ky = 0, kz = 0;
+        if (ry < 0) { ry += S; ky = 1; } else if (ry >= S) { ry -= S; ky = 2; }
+        if (rz < 0) { rz += S; kz = 1; } else if (rz >= S) { rz -= S; kz = 2; }
+        const double ys = shifted(yi, ky, g.L), zs = shifted(zi, kz, g.L);
+        const XRun xr = x_run(tab, g.c0 + S * ry + g.S2 * rz, cx, S);
+#pragma unroll
+        for (int sg = 0; sg < 2; ++sg) {
+            const int kx = sg ? xr.k1 : xr.k0;
+            cull_cell<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, shifted(xi, kx, g.L), ys, zs,
+                                          sg ? xr.b1 : xr.b0, sg ? xr.e1 : xr.e0,
                                           (unsigned)(kx + 3 * ky + 9 * kz) << kIdxBits, hits,
                                           nh, ri, fx, fy, fz, err, orig);
         }
-        drain_hits<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, ri, hits, nh, fx, fy, fz,
-                                       err, orig);
-        return;
     }
-    for (int ox = -1; ox <= 1; ++ox) {
-        int rx = cx + ox, kx = 0;
-        if (rx < 0) { rx += S; kx = 1; } else if (rx >= S) { rx -= S; kx = 2; }
-        double xs = shifted(xi, kx, g.L);
-        for (int oy = -1; oy <= 1; ++oy) {
-            int ry = cy + oy, ky = 0;
-            if (ry < 0) { ry += S; ky = 1; } else if (ry >= S) { ry -= S; ky = 2; }
-            double ys = shifted(yi, ky, g.L);
-            for (int oz = -1; oz <= 1; ++oz) {
-                int rz = cz + oz, kz = 0;
-                if (rz < 0) { rz += S; kz = 1; } else if (rz >= S) { rz -= S; kz = 2; }
-                double zs = shifted(zi, kz, g.L);
-                int h = g.c0 + rx + S * ry + g.S2 * rz;
-                int b, e;
-                tab.range(h, b, e);
-                unsigned code = (unsigned)(kx + 3 * ky + 9 * kz) << kIdxBits;
-                for (int j = b; j < e; ++j) {
-                    bool hit;
-                    if (!EXACT) {
-                        double dx = src.x(j) - xs;
-                        double dy = src.y(j) - ys;
-                        double dz = src.z(j) - zs;
-                        double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                        hit = (r2 < p.rc2_hi) && (j != s);
-                    } else {
-                        double r2 = pbc_dist_ref(xi, yi, zi, src.x(j), src.y(j),
-                                                 src.z(j), g.L, g.half_pbc);
-                        hit = (r2 < p.rc2) && (j != s);
-                    }
-                    if (hit) {
-                        hits[nh * BLOCK] = (unsigned)j | code;
-                        ++nh;
-                    }
-                    if (__any(nh == kHitCap)) {
-                        drain_hits<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, ri, hits,
-                                                       nh, fx, fy, fz, err, orig);
-                        nh = 0;
-                    }
-                }
-            }
-        }
-    }
-    drain_hits<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, ri, hits, nh, fx, fy, fz,
-                                   err, orig);
+    drain_hits<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, ri, hits, nh, fx, fy, fz, err,
+                                   orig);
 }
 
 }  // namespace psim
