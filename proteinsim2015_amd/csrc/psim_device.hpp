@@ -46,6 +46,8 @@ struct Pot {
     double ov_band;   // relative band for the exact overlap re-test
     double lj_neg_kt_eps;   // (-kT)*wellDepth
     double lj_four_n;       // 4.0*ljNum
+    double lj_att;          // (-kT)*wellDepth*4*ljNum (the kernels' folded factors)
+    double lj_rep;          // (-kT)*wellDepth*yukawaStrength
     double debye_len;
     double debye_inv;
     double yuk_a;
@@ -245,22 +247,23 @@ __device__ __forceinline__ double pair_fnet(const Pot &p, double r2, double rinv
                                             double r, double size, bool *nan_ao,
                                             const double *etab = nullptr)
 {
+    // the first enabled module's term starts the sum (no 0 + x)
     double f = 0.0;
     if (MASK & F_CAL) {
         // Calibration.cpp:75-81: -36 * rInv^37
-        f += -(36.0 * pow_const<37>(rinv));
+        f = -(36.0 * pow_const<37>(rinv));
     }
     if (MASK & F_LJ) {
-        // LJPotential.cpp:97-109
+        // LJPotential.cpp:97-109: (-kT eps) (attract + repel), the factor
+        // (-kT eps) folded into each term's constant
         const double lj = lj_power<LJN>(size * rinv, p.lj_num);
-        const double attract = fma(2.0, lj, -1.0) * (p.lj_four_n * rinv * lj);
-        double repel = 0.0;
+        double t = fma(2.0, lj, -1.0) * (p.lj_att * rinv * lj);
         if (p.yuk_a != 0.0) {
             const double ye = TAB ? exp_neg_tab(r * p.debye_inv, etab)
                                   : exp_neg(r * p.debye_inv);
-            repel = ye * (rinv * rinv * (p.debye_len + r) * p.yuk_a);
+            t = fma(ye, rinv * rinv * (p.debye_len + r) * p.lj_rep, t);
         }
-        f += p.lj_neg_kt_eps * (attract + repel);
+        f = (MASK & F_CAL) ? f + t : t;
     }
     if (MASK & F_AO) {
         // AOPotential.cpp:95-102: -(36 r^-38 + C1/r + C2 r)
@@ -268,7 +271,7 @@ __device__ __forceinline__ double pair_fnet(const Pot &p, double r2, double rinv
         const double r36 = pow_const<18>(r2inv);
         const double fa = -(fma(36.0 * r36, r2inv, fma(p.ao_c1, rinv, p.ao_c2 * r)));
         if (isnan(fa)) *nan_ao = true;
-        f += fa;
+        f = (MASK & (F_CAL | F_LJ)) ? f + fa : fa;
     }
     return f;
 }

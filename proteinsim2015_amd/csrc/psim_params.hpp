@@ -46,6 +46,8 @@ inline void make_constants(const psim_params &P, Geo &g, Pot &p, GB &b)
     p.debye_len = P.debye_length;
     p.debye_inv = 1.0 / P.debye_length;
     p.yuk_a = P.yukawa_strength;
+    p.lj_att = p.lj_neg_kt_eps * p.lj_four_n;
+    p.lj_rep = p.lj_neg_kt_eps * p.yuk_a;
     // AOPotential.cpp:50-55
     double q = (rc / (rc - 1.0));
     double a1 = -P.kT * P.ao_well_depth * q * q * q;
