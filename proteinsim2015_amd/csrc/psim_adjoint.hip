@@ -176,7 +176,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_step(
     const int *cstart, const double *lam, const double *a1, const double *a2, double *a0,
     double *lam_prev, double beta, double coeff0, double alpha_prev, double c2, double *tpart)
 {
-    const int gt = blockIdx.x * BLOCK + threadIdx.x;
+    const int gt = xcd_block() * BLOCK + threadIdx.x;
     const int s = gt / SPLIT, sub = gt % SPLIT;
     const int N = g.n;
     double th[kNTheta] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -395,7 +395,7 @@ int adj_block(const psim_ctx *) { return kAdjBlock; }
 int adj_split(const psim_ctx *c) { return c->n < kAdjSmall ? PSIM_ADJ_SPLIT : 1; }
 unsigned adj_blocks(const psim_ctx *c)
 {
-    return blocks_for((long long)c->n * adj_split(c), adj_block(c));
+    return xcd_grid(blocks_for((long long)c->n * adj_split(c), adj_block(c)));
 }
 
 // one backward step; its block partials of dL/dtheta go to tpart at tp

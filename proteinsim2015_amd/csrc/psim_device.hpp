@@ -375,4 +375,18 @@ __device__ __forceinline__ double split_sum(double v)
     return v;
 }
 
+// Workgroups are handed out round-robin over the 8 XCDs.  With a grid of
+// 8 * per workgroups (xcd_grid), workgroup b takes slab position
+// (b mod 8) * per + b / 8: each XCD walks one contiguous range of slots, and
+// the neighbour cells those slots read stay in that XCD's L2.
+__device__ __forceinline__ int xcd_block()
+{
+    const int per = (int)(gridDim.x >> 3);
+    return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+
+// Grid size for xcd_block: nb workgroups rounded up to a multiple of 8 (the
+// extra workgroups find no slot and return).
+inline unsigned xcd_grid(unsigned nb) { return 8u * ((nb + 7u) / 8u); }
+
 }  // namespace psim

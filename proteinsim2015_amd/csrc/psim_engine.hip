@@ -132,10 +132,10 @@ void launch_step_mask(psim_ctx *c, const StepArgs &a)
         // few particles: small blocks and kSmallSplit lanes per particle so
         // that every CU gets work
         k_step<MASK, 64, kSmallSplit>
-            <<<blocks_for((long long)c->n * kSmallSplit, 64), 64, 0, c->stream>>>(
+            <<<xcd_grid(blocks_for((long long)c->n * kSmallSplit, 64)), 64, 0, c->stream>>>(
                 c->geo, c->pot, c->gb, c->prm.radius, a);
     } else {
-        k_step<MASK, kBlock><<<blocks_for(c->n, kBlock), kBlock, 0, c->stream>>>(
+        k_step<MASK, kBlock><<<xcd_grid(blocks_for(c->n, kBlock)), kBlock, 0, c->stream>>>(
             c->geo, c->pot, c->gb, c->prm.radius, a);
     }
 }
@@ -173,7 +173,7 @@ void launch_forces_mask(psim_ctx *c)
             c->geo, c->pot, c->prm.radius, c->bg, per, c->pos, c->id, c->cstart, c->fout,
             c->err);
     } else {
-        k_forces<MASK><<<blocks_for(c->n, kBlock), kBlock, 0, c->stream>>>(
+        k_forces<MASK><<<xcd_grid(blocks_for(c->n, kBlock)), kBlock, 0, c->stream>>>(
             c->geo, c->pot, c->prm.radius, c->pos, c->id, c->cstart, c->fout, c->err);
     }
 }

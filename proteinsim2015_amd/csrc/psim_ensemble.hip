@@ -61,7 +61,7 @@ __global__ __launch_bounds__(BLOCK) void k_ens_step(const RepGeo *__restrict__ r
                                                     int ntot, GB gb, StepArgs a)
 {
     __shared__ unsigned hits[kHitCap * BLOCK];
-    const int s = blockIdx.x * BLOCK + threadIdx.x;
+    const int s = xcd_block() * BLOCK + threadIdx.x;
     if (s >= ntot) return;
     const int r = replica_of(p0, nrep, a.id[s]);
     const Geo g = reps[r].g;
@@ -211,7 +211,7 @@ StepArgs ens_args(psim_ens *e)
 template <int MASK, int BLOCK>
 void launch_ens_pair(psim_ens *e, const StepArgs &a)
 {
-    k_ens_step<MASK, BLOCK><<<eblocks(e->ntot, BLOCK), BLOCK, 0, e->stream>>>(
+    k_ens_step<MASK, BLOCK><<<xcd_grid(eblocks(e->ntot, BLOCK)), BLOCK, 0, e->stream>>>(
         e->reps, e->p0, e->nrep, e->ntot, e->gb, a);
 }
 

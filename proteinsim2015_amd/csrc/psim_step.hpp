@@ -170,7 +170,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(Geo g, Pot p, GB gb, double radi
 {
     static_assert(SPLIT >= 1 && 64 % SPLIT == 0 && (SPLIT & (SPLIT - 1)) == 0, "lane groups");
     __shared__ unsigned hits[kHitCap * BLOCK];
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    const int t = xcd_block() * BLOCK + threadIdx.x;
     const int s = t / SPLIT, sub = t % SPLIT;
     if (s >= g.n) return;   // whole lane groups
     const int N = g.n;
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(Geo g, Pot p, double radius,
                                                    DevErr *err)
 {
     __shared__ unsigned hits[kHitCap * kBlock];
-    const int s = blockIdx.x * kBlock + threadIdx.x;
+    const int s = xcd_block() * kBlock + threadIdx.x;
     if (s >= g.n) return;
     const int N = g.n;
     const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
