@@ -73,7 +73,7 @@ __device__ inline int brick_scan(int v, int *total, int *wsum)
 // Pair forces of one lane's hit list, entries [0, nh) (drain_hits' arithmetic
 // on staged operands).
 template <int MASK, bool EXACT>
-__device__ void brick_drain(const Geo &g, const Pot &p, const BrickShared &sh,
+__device__ __forceinline__ void brick_drain(const Geo &g, const Pot &p, const BrickShared &sh,
                             const int *__restrict__ id, double xi, double yi, double zi,
                             double size, int slot_i, int nh, double &fx, double &fy,
                             double &fz, DevErr *err)
@@ -123,7 +123,7 @@ __device__ void brick_drain(const Geo &g, const Pot &p, const BrickShared &sh,
 // Forces on the home particles of brick b.  epi(slot, fx, fy, fz) is called
 // once per home particle, by the first lane of its group.
 template <int MASK, bool EXACT, int G, class Epi>
-__device__ void brick_forces(const Geo &g, const Pot &p, double radius,
+__device__ __forceinline__ void brick_forces(const Geo &g, const Pot &p, double radius,
                              const double *__restrict__ pos, const int *__restrict__ cstart,
                              const int *__restrict__ id, DevErr *err, const BrickGeo &bg,
                              int b, BrickShared &sh, Epi epi)

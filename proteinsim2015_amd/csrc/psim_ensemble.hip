@@ -93,10 +93,12 @@ __global__ __launch_bounds__(kBrickThreads) void k_ens_step_brick(
     const int it = xcd_cell(blockIdx.x, per_xcd);
     if (it >= nitems) return;
     const int r = replica_of(item0, nrep, it);
-    const Geo g = reps[r].g;
-    const Pot p = reps[r].p;
+    // references into the replica table: local copies passed on by
+    // reference would live in scratch memory
+    const Geo &g = reps[r].g;
+    const Pot &p = reps[r].p;
     const double radius = reps[r].radius;
-    const BrickGeo bg = reps[r].bg;
+    const BrickGeo &bg = reps[r].bg;
     const int b = it - item0[r];
     auto epi = [&](int s, double fx, double fy, double fz) {
         integrate_one<true>(g, gb, a, s, fx, fy, fz);

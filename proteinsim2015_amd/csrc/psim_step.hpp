@@ -67,7 +67,7 @@ struct StepArgs {
 // (brownianIntegrator.cpp:191-356, particle.cpp:61-99); writes the next state
 // at slot s, the new cell key and an atomic per-cell rank.
 template <bool WRITE_F>
-__device__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int s,
+__device__ __forceinline__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int s,
                               double fx, double fy, double fz)
 {
     const int N = g.n;
