@@ -157,7 +157,7 @@ extern "C" int psim_init_rsa(int n, double box, double dmin, unsigned long long 
         for (long long k = 0; k < m && placed < n; ++k) {
             if (!ok[(size_t)k]) continue;
             const double *p = &cand[(size_t)k * 3];
-            if (!g.free_at(p)) continue;           // accepted earlier this round
+            if (!g.free_at(p)) continue;           // a candidate of this batch took the spot
             if (!g.insert(p)) return PSIM_E_UNSUPPORTED;
             for (int d = 0; d < 3; ++d) out[3 * (size_t)placed + d] = p[d];
             ++placed;

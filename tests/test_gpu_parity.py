@@ -1,6 +1,6 @@
 """HIP path vs the oracle and the reference's golden fixtures (needs a GPU).
 
-Tolerances (BASELINE north star): cell assignment and neighbour membership
+Tolerances: cell assignment and neighbour membership
 bit-exact; per-particle forces within 1e-6 relative of FP64 with the metric
 |dF_i| / max(|F_i|, sum_j |f_ij|); one-step positions within 1e-12 absolute."""
 import os
