@@ -115,8 +115,8 @@ __device__ __forceinline__ void pair_fnet_d(const Pot &p, const AdjPot &q, doubl
 
 // Pairs of slot s inside the cutoff, with the forward kernels' neighbour
 // test and displacement (psim_pairs.hpp gather_force / drain_hits).  Lane
-// `sub` of the SPLIT lanes that share the particle visits neighbour cells
-// sub, sub + SPLIT, ... of the 27.
+// `sub` of the SPLIT lanes that share the particle visits the x-runs sub,
+// sub + SPLIT, ... of the 9 (gather_force's walk).
 template <bool EXACT, int SPLIT, class Fn>
 __device__ void visit_pairs(const Geo &g, const Pot &p, const double *pos, const int *cstart,
                             int s, int sub, Fn fn)
@@ -125,45 +125,43 @@ __device__ void visit_pairs(const Geo &g, const Pot &p, const double *pos, const
     const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
     const int cx = cell_1d(xi, g.cell, S), cy = cell_1d(yi, g.cell, S),
               cz = cell_1d(zi, g.cell, S);
-    for (int q = sub; q < 27; q += SPLIT) {
-        {
-            int rx = cx + q / 9 - 1, kx = 0;
-            if (rx < 0) { rx += S; kx = 1; } else if (rx >= S) { rx -= S; kx = 2; }
-            const double xs = shifted(xi, kx, g.L);
-            int ry = cy + (q / 3) % 3 - 1, ky = 0;
-            if (ry < 0) { ry += S; ky = 1; } else if (ry >= S) { ry -= S; ky = 2; }
-            const double ys = shifted(yi, ky, g.L);
-            {
-                int rz = cz + q % 3 - 1, kz = 0;
-                if (rz < 0) { rz += S; kz = 1; } else if (rz >= S) { rz -= S; kz = 2; }
-                const double zs = shifted(zi, kz, g.L);
-                const int h = g.c0 + rx + S * ry + g.S2 * rz;
-                for (int j = cstart[h]; j < cstart[h + 1]; ++j) {
-                    if (j == s) continue;
-                    const double xj = pos[j], yj = pos[N + j], zj = pos[2 * N + j];
-                    double dx, dy, dz, r2, r, rinv;
-                    if (!EXACT) {
-                        dx = xj - xs;
-                        dy = yj - ys;
-                        dz = zj - zs;
-                        r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                        if (!(r2 < p.rc2_hi)) continue;
-                        if (r2 >= p.rc2_lo &&
-                            !(pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc) < p.rc2))
-                            continue;
-                        rinv = rsqrt_nr(r2);
-                        r = r2 * rinv;
-                    } else {
-                        r2 = pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc);
-                        if (!(r2 < p.rc2)) continue;
-                        r = sqrt(r2);
-                        rinv = 1.0 / r;
-                        dx = min_image_uv(xj - xi, g.L, g.half_uv);
-                        dy = min_image_uv(yj - yi, g.L, g.half_uv);
-                        dz = min_image_uv(zj - zi, g.L, g.half_uv);
-                    }
-                    fn(j, dx, dy, dz, r, rinv);
+    const StartTab tab{cstart};
+    for (int q = sub; q < 9; q += SPLIT) {
+        int ry = cy + q / 3 - 1, ky = 0;
+        if (ry < 0) { ry += S; ky = 1; } else if (ry >= S) { ry -= S; ky = 2; }
+        int rz = cz + q % 3 - 1, kz = 0;
+        if (rz < 0) { rz += S; kz = 1; } else if (rz >= S) { rz -= S; kz = 2; }
+        const double ys = shifted(yi, ky, g.L), zs = shifted(zi, kz, g.L);
+        const XRun xr = x_run(tab, g.c0 + S * ry + g.S2 * rz, cx, S);
+#pragma unroll
+        for (int sg = 0; sg < 2; ++sg) {
+            const double xs = shifted(xi, sg ? xr.k1 : xr.k0, g.L);
+            const int e = sg ? xr.e1 : xr.e0;
+            for (int j = sg ? xr.b1 : xr.b0; j < e; ++j) {
+                if (j == s) continue;
+                const double xj = pos[j], yj = pos[N + j], zj = pos[2 * N + j];
+                double dx, dy, dz, r2, r, rinv;
+                if (!EXACT) {
+                    dx = xj - xs;
+                    dy = yj - ys;
+                    dz = zj - zs;
+                    r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    if (!(r2 < p.rc2_hi)) continue;
+                    if (r2 >= p.rc2_lo &&
+                        !(pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc) < p.rc2))
+                        continue;
+                    rinv = rsqrt_nr(r2);
+                    r = r2 * rinv;
+                } else {
+                    r2 = pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc);
+                    if (!(r2 < p.rc2)) continue;
+                    r = sqrt(r2);
+                    rinv = 1.0 / r;
+                    dx = min_image_uv(xj - xi, g.L, g.half_uv);
+                    dy = min_image_uv(yj - yi, g.L, g.half_uv);
+                    dz = min_image_uv(zj - zi, g.L, g.half_uv);
                 }
+                fn(j, dx, dy, dz, r, rinv);
             }
         }
     }
