@@ -116,11 +116,16 @@ __global__ void k_scatter(int n, const int *key, const int *arank, const int *cs
 
 // Canonical order within a cell = ascending source slot (stable counting
 // sort); gathers the state into the sorted arrays.
-__global__ void k_move(int n, const int *perm, const int *key, const int *cstart,
-                       const double *npos, const double *npos0, const double *nfrc,
-                       const double *nmem, const int *nid, double *pos, double *pos0,
-                       double *frc, double *mem, int *id)
+__global__ void k_move(int n, const int *__restrict__ perm, const int *__restrict__ key,
+                       const int *__restrict__ cstart, const double *__restrict__ npos,
+                       const double *__restrict__ npos0, const double *__restrict__ nfrc,
+                       const double *__restrict__ nmem, const int *__restrict__ nid,
+                       double *__restrict__ pos, double *__restrict__ pos0,
+                       double *__restrict__ frc, double *__restrict__ mem,
+                       int *__restrict__ id)
 {
+    // the source and destination arrays are distinct buffers (restrict lets
+    // all twelve gathers issue before the first store)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int s = perm[t];
