@@ -209,7 +209,9 @@ def valu_from_profiles(pair_ms):
     rate = insts * 64 / (pair_ms * 1e-3)
     return {"achieved_lane_ops_per_s": rate, "peak": VALU_PEAK_LANE_OPS,
             "frac": rate / VALU_PEAK_LANE_OPS,
-            "note": "SQ_INSTS_VALU x 64 counts issued lanes incl. masked ones"}
+            "note": "SQ_INSTS_VALU x 64 counts issued lanes incl. masked ones; the peak "
+                    "issues one wave64 instruction per SIMD every 4 cycles (the FP64 FMA "
+                    "rate), so a frac near 1 means VALU-issue bound"}
 
 
 def main():

@@ -27,6 +27,13 @@ namespace psim {
 #ifndef PSIM_STAGE_CAP
 #define PSIM_STAGE_CAP 512
 #endif
+// Waves per SIMD the register allocation is held to: 7 leaves 72 VGPRs per
+// lane, which the kernel needs for one module (or Calibration beside one),
+// against 5 waves at the compiler's own choice of about 86; with both LJ and
+// AO enabled it needs 80 (6 waves) to stay out of scratch memory.
+#ifndef PSIM_DENSE_WAVES
+#define PSIM_DENSE_WAVES 7
+#endif
 constexpr int kDenseThreads = PSIM_DENSE_THREADS;  // lanes per home cell (4 wave64s)
 constexpr int kStageCap = PSIM_STAGE_CAP;          // staged candidates per chunk
 static_assert(kDenseThreads >= 64 && kDenseThreads % 64 == 0, "whole wave64s per cell");
@@ -48,6 +55,16 @@ struct DenseSharedT {
     double shx[27], shy[27], shz[27];
 };
 using DenseShared = DenseSharedT<kDenseThreads>;
+// waves per SIMD for the kernels of force mask `mask`
+constexpr int dense_waves(int mask)
+{
+    return ((mask & F_LJ) && (mask & F_AO)) ? (PSIM_DENSE_WAVES < 6 ? PSIM_DENSE_WAVES : 6)
+                                            : PSIM_DENSE_WAVES;
+}
+// the workgroups that PSIM_DENSE_WAVES waves per SIMD make resident on one CU
+// (4 SIMDs) must also fit its 160 KiB of LDS
+static_assert(PSIM_DENSE_WAVES * 4 * 64 / kDenseThreads * sizeof(DenseShared) <= 160 * 1024,
+              "LDS for the resident dense workgroups");
 
 // Map a workgroup index to a cell so that each XCD (workgroups are handed
 // out round-robin over 8 XCDs) walks one contiguous slab of cells and its

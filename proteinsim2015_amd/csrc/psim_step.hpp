@@ -220,12 +220,10 @@ __global__ __launch_bounds__(kBrickThreads) void k_step_brick(Geo g, Pot p, GB g
 // (Fusing the G&B epilogue here would run it on n_i of the workgroup's lanes,
 // latency-bound; the separate kernel runs it at full occupancy.)
 template <int MASK, int LJN>
-__global__ __launch_bounds__(kDenseThreads) void k_step_dense(Geo g, Pot p, double radius,
-                                                              const double *pos,
-                                                              const int *id,
-                                                              const int *cstart,
-                                                              double *nfrc, DevErr *err,
-                                                              int per_xcd)
+__global__ __launch_bounds__(kDenseThreads)
+__attribute__((amdgpu_waves_per_eu(dense_waves(MASK)))) void k_step_dense(
+    Geo g, Pot p, double radius, const double *pos, const int *id, const int *cstart,
+    double *nfrc, DevErr *err, int per_xcd)
 {
     __shared__ DenseShared sh;
     const int c = xcd_cell(blockIdx.x, per_xcd);
@@ -304,12 +302,10 @@ __global__ __launch_bounds__(kBrickThreads) void k_forces_brick(Geo g, Pot p, do
 }
 
 template <int MASK, int LJN>
-__global__ __launch_bounds__(kDenseThreads) void k_forces_dense(Geo g, Pot p, double radius,
-                                                                const double *pos,
-                                                                const int *id,
-                                                                const int *cstart,
-                                                                double *fout, DevErr *err,
-                                                                int per_xcd)
+__global__ __launch_bounds__(kDenseThreads)
+__attribute__((amdgpu_waves_per_eu(dense_waves(MASK)))) void k_forces_dense(
+    Geo g, Pot p, double radius, const double *pos, const int *id, const int *cstart,
+    double *fout, DevErr *err, int per_xcd)
 {
     __shared__ DenseShared sh;
     const int c = xcd_cell(blockIdx.x, per_xcd);
