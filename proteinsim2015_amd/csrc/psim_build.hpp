@@ -106,10 +106,59 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(int *count, int nce
     if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
 }
 
+// Cell start table in one launch for grids of at most kScanDirectTiles
+// tiles: each block sums the counts of the tiles before it itself instead of
+// reading k_scan_reduce's tile sums.  The counts stay in place (other blocks
+// are still reading them); k_scatter clears them.
+// (measured against k_scan_reduce + k_scan_apply: 7.0 vs 9.6 us at 20 tiles,
+// 9.8 vs 9.5 us at 33)
+constexpr int kScanDirectTiles = 32;
+__global__ __launch_bounds__(kScanThreads) void k_scan_direct(const int *count, int ncell,
+                                                              int *cstart, int n)
+{
+    const int lim = blockIdx.x * kScanTile;   // a multiple of 4, at most ncell
+    int before = 0;
+    // eight independent 16-byte loads in flight per lane (the sum is
+    // latency-bound otherwise)
+    constexpr int kStride = kScanThreads * 4;
+    for (int i0 = threadIdx.x * 4; i0 < lim; i0 += 8 * kStride) {
+        int4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = i0 + u * kStride;
+            v[u] = i < lim ? *reinterpret_cast<const int4 *>(count + i) : make_int4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) before += v[u].x + v[u].y + v[u].z + v[u].w;
+    }
+    int before_total;
+    block_exclusive_scan(before, &before_total);
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int c[kScanItems];
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        c[k] = (base + k < ncell) ? count[base + k] : 0;
+        v += c[k];
+    }
+    int total;
+    int pre = block_exclusive_scan(v, &total) + before_total;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < ncell) cstart[base + k] = pre;
+        pre += c[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
+}
+
+// With clear_count set (after k_scan_direct), also zeroes the ncell counts
+// for the next step's counting.
 __global__ void k_scatter(int n, const int *key, const int *arank, const int *cstart,
-                          int *perm)
+                          int *perm, int *clear_count, int ncell)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (clear_count)
+        for (int c = s; c < ncell; c += gridDim.x * blockDim.x) clear_count[c] = 0;
     if (s >= n) return;
     perm[cstart[key[s]] + arank[s]] = s;
 }
@@ -131,8 +180,14 @@ __global__ void k_move(int n, const int *__restrict__ perm, const int *__restric
     const int s = perm[t];
     const int c = key[s];
     const int b = cstart[c], e = cstart[c + 1];
+    // four independent loads per round (the count is latency-bound otherwise)
     int rank = 0;
-    for (int u = b; u < e; ++u) rank += (perm[u] < s);
+    int u = b;
+    for (; u + 4 <= e; u += 4) {
+        const int p0 = perm[u], p1 = perm[u + 1], p2 = perm[u + 2], p3 = perm[u + 3];
+        rank += (p0 < s) + (p1 < s) + (p2 < s) + (p3 < s);
+    }
+    for (; u < e; ++u) rank += (perm[u] < s);
     const int dst = b + rank;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {

@@ -15,7 +15,9 @@
 //               velocityStep), wrap (particle::setPos), new cell hash and
 //               an atomic per-cell count; writes the next state in the
 //               current slot order to the "n*" buffers.
-//   k_scan_*    counts -> cstart (exclusive scan), counts cleared.
+//   k_scan_*    counts -> cstart (exclusive scan), counts cleared (one
+//               launch, k_scan_direct, for grids of up to 32 scan tiles;
+//               k_scatter then clears the counts).
 //   k_scatter   slot of each particle in its new cell (arbitrary order).
 //   k_move      canonical within-cell order (ascending previous slot, i.e.
 //               a stable counting sort) and the state gather into the
@@ -85,9 +87,15 @@ int launch_build(psim_ctx *c)
 {
     const int n = c->n, nc = c->ncell;
     hipStream_t st = c->stream;
-    k_scan_reduce<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->bsum);
-    k_scan_apply<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->bsum, c->cstart, n);
-    k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, c->key, c->arank, c->cstart, c->perm);
+    const bool direct = c->nscan <= kScanDirectTiles;
+    if (direct) {
+        k_scan_direct<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->cstart, n);
+    } else {
+        k_scan_reduce<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->bsum);
+        k_scan_apply<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->bsum, c->cstart, n);
+    }
+    k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, c->key, c->arank, c->cstart, c->perm,
+                                                  direct ? c->count : nullptr, nc);
     k_move<<<blocks_for(n, 256), 256, 0, st>>>(n, c->perm, c->key, c->cstart, c->npos,
                                                c->npos0, c->nfrc, c->nmem, c->nid, c->pos,
                                                c->pos0, c->frc, c->mem, c->id);

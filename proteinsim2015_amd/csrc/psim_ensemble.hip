@@ -237,9 +237,15 @@ void launch_build_ens(psim_ens *e)
 {
     const int n = e->ntot, nc = e->ncell;
     hipStream_t st = e->stream;
-    k_scan_reduce<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->bsum);
-    k_scan_apply<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->bsum, e->cstart, n);
-    k_scatter<<<eblocks(n, 256), 256, 0, st>>>(n, e->key, e->arank, e->cstart, e->perm);
+    const bool direct = e->nscan <= kScanDirectTiles;
+    if (direct) {
+        k_scan_direct<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->cstart, n);
+    } else {
+        k_scan_reduce<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->bsum);
+        k_scan_apply<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->bsum, e->cstart, n);
+    }
+    k_scatter<<<eblocks(n, 256), 256, 0, st>>>(n, e->key, e->arank, e->cstart, e->perm,
+                                               direct ? e->count : nullptr, nc);
     k_move<<<eblocks(n, 256), 256, 0, st>>>(n, e->perm, e->key, e->cstart, e->npos, e->npos0,
                                             e->nfrc, e->nmem, e->nid, e->pos, e->pos0, e->frc,
                                             e->mem, e->id);
