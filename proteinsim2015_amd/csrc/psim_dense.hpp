@@ -107,7 +107,8 @@ __device__ __forceinline__ void dense_pair(const Geo &g, const Pot &p,
 
 // Survivors of one batch of n <= 32 candidates j0, j0 + G, ...: bit
 // (n - 1 - k) of m is set when candidate j0 + k G passed the cull, so taking
-// the highest bit first keeps the candidate order.
+// the highest bit first keeps the candidate order (the bit with c leading
+// zeros is candidate k = c + n - 32).
 template <int MASK, int T, int LJN>
 __device__ void dense_drain_mask(const Geo &g, const Pot &p, const DenseSharedT<T> &sh,
                                  const double *__restrict__ pos, double xi, double yi,
@@ -115,10 +116,11 @@ __device__ void dense_drain_mask(const Geo &g, const Pot &p, const DenseSharedT<
                                  int n, int j0, int G, double &fx, double &fy, double &fz,
                                  DevErr *err, int id_i, const int *__restrict__ id)
 {
+    const int jc = j0 + (n - 32) * G;
     while (m) {
-        const int b = 31 - __clz(m);
-        m ^= 1u << b;
-        const int j = j0 + (n - 1 - b) * G;
+        const int c = __clz(m);
+        m ^= 0x80000000u >> c;
+        const int j = jc + __mul24(c, G);
         if (j == self) continue;
         dense_pair<MASK, T, LJN>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, j, fx, fy, fz, err,
                                  id_i, id);

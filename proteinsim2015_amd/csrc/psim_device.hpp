@@ -177,39 +177,41 @@ __device__ __forceinline__ double exp_neg(double a)
     return ldexp(p, -(int)k);
 }
 
-// 2^(-j/64), j = 0..63, each the double nearest the exact value.
+// 2^(j/64), j = 0..63, each the double nearest the exact value.
 static __device__ const double kExp2Tab[64] = {
-    0x1.0000000000000p+0, 0x1.fa7c1819e90d8p-1, 0x1.f50765b6e4540p-1, 0x1.efa1bee615a27p-1,
-    0x1.ea4afa2a490dap-1, 0x1.e502ee78b3ff6p-1, 0x1.dfc97337b9b5fp-1, 0x1.da9e603db3285p-1,
-    0x1.d5818dcfba487p-1, 0x1.d072d4a07897cp-1, 0x1.cb720dcef9069p-1, 0x1.c67f12e57d14bp-1,
-    0x1.c199bdd85529cp-1, 0x1.bcc1e904bc1d2p-1, 0x1.b7f76f2fb5e47p-1, 0x1.b33a2b84f15fbp-1,
-    0x1.ae89f995ad3adp-1, 0x1.a9e6b5579fdbfp-1, 0x1.a5503b23e255dp-1, 0x1.a0c667b5de565p-1,
-    0x1.9c49182a3f090p-1, 0x1.97d829fde4e50p-1, 0x1.93737b0cdc5e5p-1, 0x1.8f1ae99157736p-1,
-    0x1.8ace5422aa0dbp-1, 0x1.868d99b4492edp-1, 0x1.82589994cce13p-1, 0x1.7e2f336cf4e62p-1,
-    0x1.7a11473eb0187p-1, 0x1.75feb564267c9p-1, 0x1.71f75e8ec5f74p-1, 0x1.6dfb23c651a2fp-1,
-    0x1.6a09e667f3bcdp-1, 0x1.6623882552225p-1, 0x1.6247eb03a5585p-1, 0x1.5e76f15ad2148p-1,
-    0x1.5ab07dd485429p-1, 0x1.56f4736b527dap-1, 0x1.5342b569d4f82p-1, 0x1.4f9b2769d2ca7p-1,
-    0x1.4bfdad5362a27p-1, 0x1.486a2b5c13cd0p-1, 0x1.44e086061892dp-1, 0x1.4160a21f72e2ap-1,
-    0x1.3dea64c123422p-1, 0x1.3a7db34e59ff7p-1, 0x1.371a7373aa9cbp-1, 0x1.33c08b26416ffp-1,
-    0x1.306fe0a31b715p-1, 0x1.2d285a6e4030bp-1, 0x1.29e9df51fdee1p-1, 0x1.26b4565e27cddp-1,
-    0x1.2387a6e756238p-1, 0x1.2063b88628cd6p-1, 0x1.1d4873168b9aap-1, 0x1.1a35beb6fcb75p-1,
-    0x1.172b83c7d517bp-1, 0x1.1429aaea92de0p-1, 0x1.11301d0125b51p-1, 0x1.0e3ec32d3d1a2p-1,
-    0x1.0b5586cf9890fp-1, 0x1.0874518759bc8p-1, 0x1.059b0d3158574p-1, 0x1.02c9a3e778061p-1,
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
+    0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,
+    0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0,
+    0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0,
+    0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0,
+    0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0,
+    0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0,
+    0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0,
+    0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0,
+    0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0,
 };
 
 // e^{-a} for 0 <= a < 700 with a 64-entry table (tab = kExp2Tab or a copy
-// of it): a = (64 e + j) ln2/64 + f with |f| <= ln2/128 (ln2/64 in two parts,
-// from the hi/lo split of ln 2 in fdlibm's e_exp.c),
-// e^{-f} by its Taylor series to degree 6 (truncation < 3e-20 relative),
-// times 2^{-j/64} from the table, scaled by 2^{-e}.
+// of it): -a = k ln2/64 - f with k = rint(-a 64/ln2) = 64 e + j (e = k >> 6,
+// j = k & 63, so 0 <= j < 64) and |f| <= ln2/128 (ln2/64 in two parts, from
+// the hi/lo split of ln 2 in fdlibm's e_exp.c); e^{-f} by its Taylor series
+// to degree 6 (truncation < 3e-20 relative), times 2^{j/64} from the table,
+// scaled by 2^e.  (2^{j/64} = 2 * 2^{-(64-j)/64} exactly in doubles, so this
+// rounds exactly as the same scheme with a table of 2^{-j/64} would.)
 __device__ __forceinline__ double exp_neg_tab(double a, const double *tab)
 {
-    const double inv = 92.332482616893656;            // 64 / ln 2
+    const double inv = -92.332482616893656;           // -64 / ln 2
     const double ln2_hi = 6.93147180369123816490e-01 / 64.0;
     const double ln2_lo = 1.90821492927058770002e-10 / 64.0;
-    const double k = rint(a * inv);
-    double f = fma(-k, ln2_hi, a);
-    f = fma(-k, ln2_lo, f);
+    const double k = rint(a * inv);   // <= 0
+    double f = fma(k, ln2_hi, a);
+    f = fma(k, ln2_lo, f);
     const double x = -f;
     double q = 1.0 / 720.0;
     q = fma(q, x, 1.0 / 120.0);
@@ -219,7 +221,7 @@ __device__ __forceinline__ double exp_neg_tab(double a, const double *tab)
     q = fma(q, x, 1.0);
     q = fma(q, x, 1.0);
     const int ki = (int)k;
-    return ldexp(q * tab[ki & 63], -(ki >> 6));
+    return ldexp(q * tab[ki & 63], ki >> 6);
 }
 
 // (size/r)^n for the LJ module, n = ljNum; the common exponents unrolled.
@@ -239,7 +241,7 @@ __device__ __forceinline__ double lj_power(double sr, int n)
 
 // Pair force magnitude (positive = attractive) summed over the enabled
 // modules.  r2, rinv = 1/sqrt(r2), r = r2*rinv; size = r_i + r_j.  TAB: the
-// Yukawa exponential through the 2^(-j/64) table at etab (kernels that keep
+// Yukawa exponential through the 2^(j/64) table at etab (kernels that keep
 // a copy in LDS).  LJN: ljNum fixed at compile time (0: p.lj_num).
 // Sets *nan_ao when the AO term is NaN (AOPotential.cpp:105-109).
 template <int MASK, bool TAB = false, int LJN = 0>
