@@ -71,7 +71,7 @@ AdjPot make_adj(const psim_params &P)
 }
 
 // fNet(r), dfNet/dr and dfNet/dtheta of the enabled modules (the forward
-// expressions of pair_fnet, LJPotential.cpp:97-109, AOPotential.cpp:95-102,
+// expressions of pair_fr, LJPotential.cpp:97-109, AOPotential.cpp:95-102,
 // Calibration.cpp:75-81).
 template <int MASK>
 __device__ __forceinline__ void pair_fnet_d(const Pot &p, const AdjPot &q, double r,
@@ -90,7 +90,8 @@ __device__ __forceinline__ void pair_fnet_d(const Pot &p, const AdjPot &q, doubl
     }
     if (MASK & F_LJ) {
         const double n = (double)p.lj_num;
-        const double lj = lj_power(size * rinv, p.lj_num);
+        const double sr = size * rinv;
+        const double lj = lj_power_x(sr * sr, p.lj_num, size, rinv);
         const double attract = fma(2.0, lj, -1.0) * (p.lj_four_n * rinv * lj);
         const double dattract = -p.lj_four_n * lj * r2inv * (n * fma(4.0, lj, -1.0) + fma(2.0, lj, -1.0));
         const double lam = p.debye_len;

@@ -13,7 +13,7 @@
 // (the 27 neighbour cells of LJPotential.cpp:154-177 in gather_force's
 // order: (dy, dz) y outer, then dx = -1, 0, +1), candidates in slot order, with
 // k_step's arithmetic (shifted home position per neighbour cell, guard-banded
-// r^2, util::pbcDist's recheck, pair_fnet), survivors through a per-lane LDS
+// r^2, util::pbcDist's recheck, pair_fr), survivors through a per-lane LDS
 // hit list, and the G partials summed by the same xor butterfly.  A brick
 // step therefore equals a k_step step with SPLIT = G bit for bit; only where
 // the operands come from changes.
@@ -111,9 +111,8 @@ __device__ __forceinline__ void brick_drain(const Geo &g, const Pot &p, const Br
             if (sqrt(r2x) < ov) report_error(err, 7704, id[slot_i], id[sh.sslot[j]]);
         }
         bool nan_ao = false;
-        const double f = pair_fnet<MASK>(p, r2, rinv, r, size, &nan_ao);
+        const double fr = pair_fr<MASK>(p, rinv, r, size, &nan_ao);
         if (nan_ao) report_error(err, 7705, id[slot_i], id[sh.sslot[j]]);
-        const double fr = f * rinv;
         fx = fma(fr, dx, fx);
         fy = fma(fr, dy, fy);
         fz = fma(fr, dz, fz);

@@ -97,9 +97,8 @@ __device__ __forceinline__ void dense_pair(const Geo &g, const Pot &p,
     const double rinv = rsqrt_nr(r2);
     const double r = r2 * rinv;
     bool nan_ao = false;
-    const double f = pair_fnet<MASK, true, LJN>(p, r2, rinv, r, size, &nan_ao, sh.etab);
+    const double fr = pair_fr<MASK, true, LJN>(p, rinv, r, size, &nan_ao, sh.etab);
     if (nan_ao) report_error(err, 7705, id_i, id[sh.sslot[j]]);
-    const double fr = f * rinv;
     fx = fma(fr, dx, fx);
     fy = fma(fr, dy, fy);
     fz = fma(fr, dz, fz);

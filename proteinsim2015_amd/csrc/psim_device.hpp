@@ -224,54 +224,57 @@ __device__ __forceinline__ double exp_neg_tab(double a, const double *tab)
     return ldexp(q * tab[ki & 63], ki >> 6);
 }
 
-// (size/r)^n for the LJ module, n = ljNum; the common exponents unrolled.
-// LJN > 0 fixes n at compile time (same operations, no branch on n).
+// (size/r)^n for the LJ module, n = ljNum: from x = (size/r)^2 for the common
+// even exponents (powBinaryDecomp of size * rinv otherwise, utilities.cpp:
+// 144-154); LJN > 0 fixes n at compile time (no branch on n).
 template <int LJN = 0>
-__device__ __forceinline__ double lj_power(double sr, int n)
+__device__ __forceinline__ double lj_power_x(double x, int n, double size, double rinv)
 {
-    const double x = sr * sr;
     switch (LJN > 0 ? LJN : n) {
     case 6: return x * x * x;
     case 12: { double x2 = x * x; double x4 = x2 * x2; return x4 * x2; }
     case 18: { double x2 = x * x; double x4 = x2 * x2; double x8 = x4 * x4; return x8 * x; }
     case 24: { double x2 = x * x; double x4 = x2 * x2; double x8 = x4 * x4; return x8 * x4; }
-    default: return pow_int(sr, n);
+    default: return pow_int(size * rinv, n);
     }
 }
 
-// Pair force magnitude (positive = attractive) summed over the enabled
-// modules.  r2, rinv = 1/sqrt(r2), r = r2*rinv; size = r_i + r_j.  TAB: the
-// Yukawa exponential through the 2^(j/64) table at etab (kernels that keep
-// a copy in LDS).  LJN: ljNum fixed at compile time (0: p.lj_num).
-// Sets *nan_ao when the AO term is NaN (AOPotential.cpp:105-109).
+// fNet / r, the pair force magnitude (positive = attractive) summed over the
+// enabled modules and divided by r: the kernels need only the force vector
+// (fNet / r) (x_j - x_i).  r^-2 is shared between the terms and the division
+// by r folded into each term.  rinv = 1/r; size = r_i + r_j.  TAB: the Yukawa
+// exponential through the 2^(j/64) table at etab (kernels that keep a copy
+// in LDS).  LJN: ljNum fixed at compile time (0: p.lj_num).  Sets *nan_ao
+// when the AO term is NaN (AOPotential.cpp:105-109).
 template <int MASK, bool TAB = false, int LJN = 0>
-__device__ __forceinline__ double pair_fnet(const Pot &p, double r2, double rinv,
-                                            double r, double size, bool *nan_ao,
-                                            const double *etab = nullptr)
+__device__ __forceinline__ double pair_fr(const Pot &p, double rinv, double r, double size,
+                                          bool *nan_ao, const double *etab = nullptr)
 {
+    const double r2inv = rinv * rinv;
     // the first enabled module's term starts the sum (no 0 + x)
     double f = 0.0;
     if (MASK & F_CAL) {
-        // Calibration.cpp:75-81: -36 * rInv^37
-        f = -(36.0 * pow_const<37>(rinv));
+        // Calibration.cpp:75-81: fNet = -36 rInv^37
+        f = -(36.0 * pow_const<19>(r2inv));
     }
     if (MASK & F_LJ) {
-        // LJPotential.cpp:97-109: (-kT eps) (attract + repel), the factor
-        // (-kT eps) folded into each term's constant
-        const double lj = lj_power<LJN>(size * rinv, p.lj_num);
-        double t = fma(2.0, lj, -1.0) * (p.lj_att * rinv * lj);
+        // LJPotential.cpp:97-109: fNet = (-kT eps) (attract + repel) with
+        // attract = (2 lj - 1) 4 n lj / r, repel = A e^{-r/lambda} (lambda + r)
+        // / r^2; the factor (-kT eps) folded into each term's constant
+        const double lj = lj_power_x<LJN>((size * size) * r2inv, p.lj_num, size, rinv);
+        double t = fma(2.0, lj, -1.0) * (p.lj_att * lj);
         if (p.yuk_a != 0.0) {
             const double ye = TAB ? exp_neg_tab(r * p.debye_inv, etab)
                                   : exp_neg(r * p.debye_inv);
-            t = fma(ye, rinv * rinv * (p.debye_len + r) * p.lj_rep, t);
+            t = fma(ye, rinv * (p.debye_len + r) * p.lj_rep, t);
         }
+        t *= r2inv;
         f = (MASK & F_CAL) ? f + t : t;
     }
     if (MASK & F_AO) {
-        // AOPotential.cpp:95-102: -(36 r^-38 + C1/r + C2 r)
-        const double r2inv = rinv * rinv;
-        const double r36 = pow_const<18>(r2inv);
-        const double fa = -(fma(36.0 * r36, r2inv, fma(p.ao_c1, rinv, p.ao_c2 * r)));
+        // AOPotential.cpp:95-102: fNet = -(36 r^-38 + C1 / r + C2 r)
+        const double r38 = pow_const<19>(r2inv);
+        const double fa = -(fma(36.0 * r38, rinv, fma(p.ao_c1, r2inv, p.ao_c2)));
         if (isnan(fa)) *nan_ao = true;
         f = (MASK & (F_CAL | F_LJ)) ? f + fa : fa;
     }

@@ -166,9 +166,8 @@ __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
             if (sqrt(r2x) < ov) report_error(err, 7704, orig(s), orig(j));
         }
         bool nan_ao = false;
-        double f = pair_fnet<MASK>(p, r2, rinv, r, size, &nan_ao);
+        const double fr = pair_fr<MASK>(p, rinv, r, size, &nan_ao);
         if (nan_ao) report_error(err, 7705, orig(s), orig(j));
-        double fr = f * rinv;
         fx = fma(fr, dx, fx);
         fy = fma(fr, dy, fy);
         fz = fma(fr, dz, fz);
