@@ -170,6 +170,21 @@ int psim_gpu_force_table(psim_ctx *ctx, const double *sorted4,
                          const int *hash_index, const int *cell_start_end,
                          double *force_out);
 
+/* Neighbour sets of the current positions as the pair kernels see them: for
+ * each particle (original index) the number of partners its pair test
+ * accepts (the strict r^2 < cutOff^2 of LJPotential.cpp:82 over the 27-cell
+ * stencil of LJPotential.cpp:139-177 with util::pbcDist,
+ * utilities.cpp:79-104) and the sum mod 2^64 of h(partner's original
+ * index), h = the SplitMix64 finaliser of (index + 0x9E3779B97F4A7C15).
+ * which = PSIM_PAIRS_STEP runs the acceptance code and launch shape of the
+ * pair kernel the context steps with; PSIM_PAIRS_ADJOINT that of the
+ * trajectory adjoint's backward step.  No state change.  Replaces no
+ * reference call: it exposes what iterCells (LJPotential.cpp:64-133)
+ * computes internally, so the neighbour sets can be compared exactly. */
+enum { PSIM_PAIRS_STEP = 0, PSIM_PAIRS_ADJOINT = 1 };
+int psim_gpu_pair_stats(psim_ctx *ctx, int which, int *count_by_orig,
+                        unsigned long long *hash_by_orig);
+
 /* Cell data of the current positions for inspection: key_by_orig[n]
  * (cell hash per original index), cell_start[S^3+1] (slot offsets),
  * orig_by_slot[n] (original index stored at each slot). */
@@ -230,6 +245,11 @@ int psim_ens_set_graph(psim_ens *ens, int steps_per_graph);
 int psim_ens_step(psim_ens *ens, int nsteps);
 int psim_ens_sync(psim_ens *ens);
 int psim_ens_last_error(psim_ens *ens, int *code, int *i, int *j);
+/* Neighbour sets of every replica as the ensemble's current pair kernel sees
+ * them (count and partner-hash sum per particle, by concatenated original
+ * index, partners named by their concatenated original index; as
+ * psim_gpu_pair_stats). */
+int psim_ens_pair_stats(psim_ens *ens, int *count_by_orig, unsigned long long *hash_by_orig);
 /* measured choice of the pair kernel: *kernel 1 = thread per particle with
  * workgroup size *tile (64/128/256), 3 = bricks of cells per workgroup with
  * about *tile home particles per brick (96/192/384); the trial steps advance

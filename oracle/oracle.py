@@ -51,6 +51,7 @@ class GBCoeffs(C.Structure):
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 
 
 def _f64(a):
@@ -75,6 +76,9 @@ class Oracle:
                                 C.c_int, C.c_double, C.c_double, C.c_double, _dp,
                                 _ip, C.c_int]
         L.po_forces.restype = C.c_int
+        L.po_pair_stats.argtypes = [C.POINTER(ForceParams), C.c_int, _dp, _ip, _ip, _ip,
+                                    C.c_int, C.c_double, C.c_double, _ip, _up, C.c_void_p,
+                                    C.c_int, C.c_int]
         L.po_gb_coeffs_init.argtypes = [C.c_double] * 4 + [C.POINTER(GBCoeffs)]
         L.po_gb_first_step.argtypes = [C.POINTER(GBCoeffs), C.c_int, C.c_double,
                                        _dp, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -168,6 +172,26 @@ class Oracle:
         sorted4 = self.reorder(perm, pos, radius)
         return self.forces_table(fp, sorted4, perm, cs, ce, S, cell_size, box,
                                  half_uv, nthreads)
+
+    def pair_stats(self, fp, pos, S, cell_size, box, radius=0.5, abs_sum=False, id0=0,
+                   nthreads=0):
+        """Per-particle partner count and partner-hash sum (uint64) of the
+        force loop's neighbour sets on a fresh cell table, by original index;
+        with abs_sum, also sum_j |fNet(r_ij)|.  Partners are named by original
+        index + id0."""
+        pos = _f64(pos).reshape(-1, 3)
+        n = len(pos)
+        key = self.hash(pos, cell_size, S)
+        perm, cs, ce = self.build_cells(key, S ** 3)
+        sorted4 = self.reorder(perm, pos, radius)
+        cnt = np.empty(n, np.int32)
+        hs = np.empty(n, np.uint64)
+        fs = np.empty(n) if abs_sum else None
+        self.lib.po_pair_stats(C.byref(fp), n, sorted4.ravel(), perm, cs, ce, int(S),
+                               float(cell_size), float(box), cnt, hs,
+                               None if fs is None else fs.ctypes.data_as(C.c_void_p),
+                               int(id0), int(nthreads))
+        return (cnt, hs, fs) if abs_sum else (cnt, hs)
 
     # -- integrator ---------------------------------------------------------
     def gb_coeffs(self, kT, mass, gamma, dt):

@@ -306,6 +306,77 @@ int po_forces(const po_force_params *fp, int n, const double *sorted4,
 }
 
 /* ------------------------------------------------------------------ */
+/* Neighbour sets of the force loop: getAcceleration's 27-cell stencil   */
+/* (LJPotential.cpp:139-177) and iterCells' test (:64-95: skip i ==      */
+/* index, util::pbcDist, strict rSquared < cutOff^2).                    */
+/* ------------------------------------------------------------------ */
+static inline unsigned long long po_nbr_hash(unsigned o)
+{
+    /* SplitMix64 finaliser (Steele, Lea and Flood, OOPSLA 2014) */
+    unsigned long long z = (unsigned long long)o + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void po_pair_stats(const po_force_params *fp, int n, const double *sorted4,
+                   const int *perm, const int *cstart, const int *cend,
+                   int S, double cell_size, double box, int *count,
+                   unsigned long long *hash, double *abs_sum, int id0, int nthreads)
+{
+    po_pair_consts k;
+    po_pair_consts_init(fp, &k);
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int index = 0; index < n; index++) {
+        const double *a = sorted4 + 4 * index;
+        int cx = po_cell_1d(a[0], cell_size, S);
+        int cy = po_cell_1d(a[1], cell_size, S);
+        int cz = po_cell_1d(a[2], cell_size, S);
+        int S2 = S * S;
+        int cnt = 0;
+        unsigned long long hs = 0;
+        double fsum = 0.0;
+        for (int x = -1; x <= 1; x++) {
+            for (int y = -1; y <= 1; y++) {
+                for (int z = -1; z <= 1; z++) {
+                    int rx = (cx + x) % S, ry = (cy + y) % S, rz = (cz + z) % S;
+                    rx = (rx < 0) ? rx + S : rx;
+                    ry = (ry < 0) ? ry + S : ry;
+                    rz = (rz < 0) ? rz + S : rz;
+                    int hsh = rx + (S * ry) + (S2 * rz);
+                    int start = cstart[hsh];
+                    if (start == -1) continue;
+                    for (int i = start; i < cend[hsh]; i++) {
+                        if (i == index) continue;
+                        const double *b = sorted4 + 4 * i;
+                        double r2 = po_pbc_dist(a[0], a[1], a[2], b[0], b[1], b[2], box);
+                        if (!(r2 < k.rc2)) continue;
+                        cnt++;
+                        hs += po_nbr_hash((unsigned)(perm[i] + id0));
+                        if (abs_sum) {
+                            double r = sqrt(r2), size = a[3] + b[3], f = 0.0;
+                            if (fp->mask & PO_CAL) f += po_f_cal(r);
+                            if (fp->mask & PO_LJ) f += po_f_lj(fp, &k, r, size);
+                            if (fp->mask & PO_AO) f += po_f_ao(&k, r, r2);
+                            fsum += fabs(f);
+                        }
+                    }
+                }
+            }
+        }
+        int o = perm[index];
+        count[o] = cnt;
+        hash[o] = hs;
+        if (abs_sum) abs_sum[o] = fsum;
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* G&B integrator coefficients: brownianIntegrator.cpp:53-81, 111-175    */
 /* ------------------------------------------------------------------ */
 void po_gb_coeffs_init(double kT, double mass, double gamma, double dt,

@@ -17,6 +17,9 @@
  *                          reference harness cell table (set equality)  pinned
  *   po_forces              bit-exact vs compiled reference plugins fed
  *                          the same cell table; two-particle closed forms pinned
+ *   po_pair_stats          brute-force O(N^2) minimum-image partner sets
+ *                          with pbcDist's arithmetic; abs_sum vs the
+ *                          closed-form pair forces                       pinned
  *   po_gb_*                bit-exact vs compiled reference integrator;
  *                          series cross-check at small y                pinned
  *   po_fastrandom_streams  bit-exact vs compiled reference fastRandom    pinned
@@ -96,6 +99,18 @@ int po_forces(const po_force_params *fp, int n, const double *sorted4,
               const int *perm, const int *cstart, const int *cend,
               int S, double cell_size, double box, double half_uv,
               double *force3, int *err_ij, int nthreads);
+
+/* Neighbour sets of po_forces' loop (LJPotential.cpp:64-95, 139-177: the
+ * 27-cell stencil, j != index, util::pbcDist, strict r^2 < cutOff^2): per
+ * particle (ORIGINAL index) the partner count, the sum mod 2^64 of
+ * h(partner's original index) with h the SplitMix64 finaliser of
+ * (index + 0x9E3779B97F4A7C15), and (abs_sum, may be NULL) the sum over
+ * partners of |fNet| of the enabled modules together.  Partners are named by
+ * original index + id0 (a replica's offset in a stacked ensemble). */
+void po_pair_stats(const po_force_params *fp, int n, const double *sorted4,
+                   const int *perm, const int *cstart, const int *cend,
+                   int S, double cell_size, double box, int *count,
+                   unsigned long long *hash, double *abs_sum, int id0, int nthreads);
 
 /* brownianIntegrator.cpp:53-81 + setupHigh/Low/Zero :111-175. */
 void po_gb_coeffs_init(double kT, double mass, double gamma, double dt,

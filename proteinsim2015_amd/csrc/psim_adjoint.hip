@@ -397,7 +397,29 @@ unsigned adj_blocks(const psim_ctx *c)
     return xcd_grid(blocks_for((long long)c->n * adj_split(c), adj_block(c)));
 }
 
-// one backward step; its block partials of dL/dtheta go to tpart at tp
+// Neighbour-set statistics of visit_pairs with the backward step's launch
+// shape (each accepted pair counted; see NbrStats).
+template <bool EXACT, int BLOCK, int SPLIT>
+__global__ __launch_bounds__(BLOCK) void k_adj_stats(Geo g, Pot p, const double *pos,
+                                                     const int *id, const int *cstart,
+                                                     NbrStatsOut so)
+{
+    const int gt = xcd_block() * BLOCK + threadIdx.x;
+    const int s = gt / SPLIT, sub = gt % SPLIT;
+    if (s >= g.n) return;   // whole lane groups
+    NbrStats st;
+    visit_pairs<EXACT, SPLIT>(g, p, pos, cstart, s, sub,
+                              [&](int j, double, double, double, double, double) {
+                                  st.add(id[j]);
+                              });
+    split_stats<SPLIT>(st);
+    if (sub == 0) {
+        so.count[id[s]] = st.count;
+        so.hash[id[s]] = st.hash;
+    }
+}
+
+// one backward step; its block partials of dL/dtheta go to tp
 template <int MASK>
 void launch_adj_mask(psim_adjoint *A, const AdjPot &q, const double *pos, const int *id,
                      const int *cs, const double *lam, const double *a1, const double *a2,
@@ -421,6 +443,29 @@ void launch_adj_mask(psim_adjoint *A, const AdjPot &q, const double *pos, const 
 }
 
 }  // namespace
+
+namespace psim {
+namespace eng {
+
+void launch_adj_stats(psim_ctx *c, NbrStatsOut so)
+{
+    const unsigned nb = adj_blocks(c);
+    const bool split = adj_split(c) > 1;
+#define PSIM_ADJ_STATS(EX, SP)                                                             \
+    k_adj_stats<EX, kAdjBlock, SP><<<nb, kAdjBlock, 0, c->stream>>>(c->geo, c->pot, c->pos, \
+                                                                    c->id, c->cstart, so)
+    if (c->geo.exact_mi) {
+        if (split) PSIM_ADJ_STATS(true, PSIM_ADJ_SPLIT);
+        else PSIM_ADJ_STATS(true, 1);
+    } else {
+        if (split) PSIM_ADJ_STATS(false, PSIM_ADJ_SPLIT);
+        else PSIM_ADJ_STATS(false, 1);
+    }
+#undef PSIM_ADJ_STATS
+}
+
+}  // namespace eng
+}  // namespace psim
 
 extern "C" {
 

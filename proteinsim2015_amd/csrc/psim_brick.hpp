@@ -72,11 +72,11 @@ __device__ inline int brick_scan(int v, int *total, int *wsum)
 
 // Pair forces of one lane's hit list, entries [0, nh) (drain_hits' arithmetic
 // on staged operands).
-template <int MASK, bool EXACT>
+template <int MASK, bool EXACT, class ST>
 __device__ __forceinline__ void brick_drain(const Geo &g, const Pot &p, const BrickShared &sh,
                             const int *__restrict__ id, double xi, double yi, double zi,
                             double size, int slot_i, int nh, double &fx, double &fy,
-                            double &fz, DevErr *err)
+                            double &fz, DevErr *err, ST &st)
 {
     const int lane = threadIdx.x;
     for (int k = 0; k < nh; ++k) {
@@ -105,6 +105,10 @@ __device__ __forceinline__ void brick_drain(const Geo &g, const Pot &p, const Br
             dy = min_image_uv(yj - yi, g.L, g.half_uv);
             dz = min_image_uv(zj - zi, g.L, g.half_uv);
         }
+        if constexpr (ST::kOn) {
+            st.add(id[sh.sslot[j]]);
+            continue;
+        }
         const double ov = p.ov_factor * size;
         if (r2 < ov * ov * (1.0 + p.ov_band)) {
             const double r2x = pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc);
@@ -120,12 +124,15 @@ __device__ __forceinline__ void brick_drain(const Geo &g, const Pot &p, const Br
 }
 
 // Forces on the home particles of brick b.  epi(slot, fx, fy, fz) is called
-// once per home particle, by the first lane of its group.
-template <int MASK, bool EXACT, int G, class Epi>
+// once per home particle, by the first lane of its group.  With ST =
+// NbrStats the accepted pairs are counted instead and written to *so by
+// original index (epi is not called).
+template <int MASK, bool EXACT, int G, class ST = NoStats, class Epi>
 __device__ __forceinline__ void brick_forces(const Geo &g, const Pot &p, double radius,
                              const double *__restrict__ pos, const int *__restrict__ cstart,
                              const int *__restrict__ id, DevErr *err, const BrickGeo &bg,
-                             int b, BrickShared &sh, Epi epi)
+                             int b, BrickShared &sh, Epi epi,
+                             const NbrStatsOut *so = nullptr)
 {
     static_assert(G >= 1 && 64 % G == 0 && (G & (G - 1)) == 0, "lane groups");
     constexpr int T = kBrickThreads;
@@ -215,6 +222,7 @@ __device__ __forceinline__ void brick_forces(const Geo &g, const Pot &p, double 
             w = lo / ey + 1;
         }
         double fx = 0.0, fy = 0.0, fz = 0.0;
+        ST st;
         for (int s0 = 0; s0 < M; s0 += kBrickStageCap) {
             const int mc = min(kBrickStageCap, M - s0);
             if (s0 != resident) {
@@ -278,15 +286,23 @@ __device__ __forceinline__ void brick_forces(const Geo &g, const Pot &p, double 
                             }
                             if (__any(nh == kBrickHitCap)) {
                                 brick_drain<MASK, EXACT>(g, p, sh, id, xi, yi, zi, size, slot,
-                                                         nh, fx, fy, fz, err);
+                                                         nh, fx, fy, fz, err, st);
                                 nh = 0;
                             }
                         }
                     }
                 }
                 brick_drain<MASK, EXACT>(g, p, sh, id, xi, yi, zi, size, slot, nh, fx, fy, fz,
-                                         err);
+                                         err, st);
             }
+        }
+        if constexpr (ST::kOn) {
+            split_stats<G>(st);
+            if (act && gg == 0) {
+                so->count[id[slot]] = st.count;
+                so->hash[id[slot]] = st.hash;
+            }
+            continue;
         }
         fx = split_sum<G>(fx);
         fy = split_sum<G>(fy);

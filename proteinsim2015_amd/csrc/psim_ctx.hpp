@@ -35,6 +35,8 @@ struct psim_ctx {
     size_t noise_cap = 0;
     double *tsorted = nullptr;
     int *thi = nullptr, *tcse = nullptr;
+    int *st_count = nullptr;                 // neighbour-set statistics
+    unsigned long long *st_hash = nullptr;
     double time = 0.0;
     int vel_counter = 0;
     unsigned long long step_index = 0;
@@ -93,6 +95,9 @@ int launch_build(psim_ctx *c);
 // nsteps of system::run with the context's step bookkeeping
 int run_steps(psim_ctx *c, int nsteps, const double *d_noise);
 int read_error(psim_ctx *c, int *code, int *i, int *j);
+// neighbour-set statistics of the adjoint's pair visitor (launch shape of
+// its backward step)
+void launch_adj_stats(psim_ctx *c, NbrStatsOut so);
 // psim_gpu_set_state with an explicit staging slot order (null = identity)
 int set_state(psim_ctx *c, const double *pos, const double *pos0, const double *vel,
               const double *frc, const double *memx, double current_time, int vel_counter,

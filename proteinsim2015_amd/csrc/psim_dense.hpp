@@ -76,13 +76,13 @@ __device__ inline int xcd_cell(int b, int per_xcd)
 
 // Pair force of staged candidate j on the lane's home particle (FP64; the
 // exact pbcDist recheck for survivors inside the guard band).
-template <int MASK, int T, int LJN>
+template <int MASK, int T, int LJN, class ST>
 __device__ __forceinline__ void dense_pair(const Geo &g, const Pot &p,
                                            const DenseSharedT<T> &sh,
                                            const double *__restrict__ pos, double xi, double yi,
                                            double zi, double size, double ov2_hi, int j,
                                            double &fx, double &fy, double &fz, DevErr *err,
-                                           int id_i, const int *__restrict__ id)
+                                           int id_i, const int *__restrict__ id, ST &st)
 {
     const int N = g.n;
     const double dx = sh.sx[j] - xi, dy = sh.sy[j] - yi, dz = sh.sz[j] - zi;
@@ -92,7 +92,11 @@ __device__ __forceinline__ void dense_pair(const Geo &g, const Pot &p,
         const double r2x = pbc_dist_ref(xi, yi, zi, pos[sj], pos[N + sj], pos[2 * N + sj],
                                         g.L, g.half_pbc);
         if (r2 >= p.rc2_lo && !(r2x < p.rc2)) return;
-        if (sqrt(r2x) < p.ov_factor * size) report_error(err, 7704, id_i, id[sj]);
+        if (!ST::kOn && sqrt(r2x) < p.ov_factor * size) report_error(err, 7704, id_i, id[sj]);
+    }
+    if constexpr (ST::kOn) {
+        st.add(id[sh.sslot[j]]);
+        return;
     }
     const double rinv = rsqrt_nr(r2);
     const double r = r2 * rinv;
@@ -108,12 +112,12 @@ __device__ __forceinline__ void dense_pair(const Geo &g, const Pot &p,
 // (n - 1 - k) of m is set when candidate j0 + k G passed the cull, so taking
 // the highest bit first keeps the candidate order (the bit with c leading
 // zeros is candidate k = c + n - 32).
-template <int MASK, int T, int LJN>
+template <int MASK, int T, int LJN, class ST>
 __device__ void dense_drain_mask(const Geo &g, const Pot &p, const DenseSharedT<T> &sh,
                                  const double *__restrict__ pos, double xi, double yi,
                                  double zi, double size, double ov2_hi, int self, unsigned m,
                                  int n, int j0, int G, double &fx, double &fy, double &fz,
-                                 DevErr *err, int id_i, const int *__restrict__ id)
+                                 DevErr *err, int id_i, const int *__restrict__ id, ST &st)
 {
     const int jc = j0 + (n - 32) * G;
     while (m) {
@@ -122,19 +126,22 @@ __device__ void dense_drain_mask(const Geo &g, const Pot &p, const DenseSharedT<
         const int j = jc + __mul24(c, G);
         if (j == self) continue;
         dense_pair<MASK, T, LJN>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, j, fx, fy, fz, err,
-                                 id_i, id);
+                                 id_i, id, st);
     }
 }
 
 // Forces on the home particles of cell c.  On return, lanes 0..ni-1 hold the
 // total force on home particle (i0 + lane) for each i-chunk via the callback
-// epi(i, fx, fy, fz) (called by those lanes only).
-template <int MASK, int T, int LJN, class Epi>
+// epi(i, fx, fy, fz) (called by those lanes only).  With ST = NbrStats the
+// accepted pairs are counted instead and written to *so by original index
+// (epi is not called).
+template <int MASK, int T, int LJN, class ST = NoStats, class Epi>
 __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
                                   const double *__restrict__ pos,
                                   const int *__restrict__ cstart,
                                   const int *__restrict__ id, DevErr *err, int c,
-                                  DenseSharedT<T> &sh, Epi epi)
+                                  DenseSharedT<T> &sh, Epi epi,
+                                  const NbrStatsOut *so = nullptr)
 {
     const int N = g.n, S = g.S;
     const int lane = threadIdx.x;
@@ -195,6 +202,7 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
         const float xf = (float)(xi - ox), yf = (float)(yi - oy), zf = (float)(zi - oz);
         const int id_i = id[si];
         double fx = 0.0, fy = 0.0, fz = 0.0;
+        ST st;
         for (int s0 = 0; s0 < M; s0 += kStageCap) {
             const int mc = min(kStageCap, M - s0);
             __syncthreads();   // the previous chunk is no longer read
@@ -245,9 +253,28 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
                     }
                     dense_drain_mask<MASK, T, LJN>(g, p, sh, pos, xi, yi, zi, size, ov2_hi,
                                                    hi - s0, m, n, j0, G, fx, fy, fz, err, id_i,
-                                                   id);
+                                                   id, st);
                 }
             }
+        }
+        if constexpr (ST::kOn) {
+            // ---- sum the G group counts of each home particle
+            __syncthreads();   // the cull's copy is no longer read
+            sh.u.red[0][lane] = (double)(act ? st.count : 0);
+            sh.u.red[1][lane] = __longlong_as_double((long long)(act ? st.hash : 0ull));
+            __syncthreads();
+            if (lane < nic) {
+                int pairs = 0;
+                unsigned long long hs = 0;
+                for (int q = 0; q < G; ++q) {
+                    pairs += (int)sh.u.red[0][lane + q * nic];
+                    hs += (unsigned long long)__double_as_longlong(sh.u.red[1][lane + q * nic]);
+                }
+                so->count[id_i] = pairs;
+                so->hash[id_i] = hs;
+            }
+            __syncthreads();
+            continue;
         }
         // ---- sum the G group partials of each home particle
         __syncthreads();   // the cull's copy is no longer read

@@ -81,6 +81,57 @@ __device__ inline void report_error(DevErr *e, int code, int i, int j)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Neighbour-set statistics.  A pair kernel instantiated with NbrStats instead
+// of NoStats records, for every particle, the pairs its acceptance test
+// admits (the strict r^2 < cutOff^2 of LJPotential.cpp:82 after the
+// guard-banded cull and util::pbcDist's recheck) and skips the force
+// arithmetic: the count, and the sum mod 2^64 of a mixing hash of each
+// partner's original index (the finaliser of the SplitMix64 generator of
+// Steele, Lea and Flood, OOPSLA 2014), which differs between two partner
+// sets of equal size unless the hashes collide.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline unsigned long long nbr_hash(unsigned o)
+{
+    unsigned long long z = (unsigned long long)o + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct NoStats {
+    static constexpr bool kOn = false;
+    __device__ void add(int) {}
+};
+
+struct NbrStats {
+    static constexpr bool kOn = true;
+    int count = 0;
+    unsigned long long hash = 0;
+    __device__ void add(int o)
+    {
+        ++count;
+        hash += nbr_hash((unsigned)o);
+    }
+};
+
+// Where a statistics kernel writes, by original particle index.
+struct NbrStatsOut {
+    int *count;
+    unsigned long long *hash;
+};
+
+// Sum of the statistics of the SPLIT consecutive lanes that share a particle.
+template <int SPLIT>
+__device__ __forceinline__ void split_stats(NbrStats &st)
+{
+#pragma unroll
+    for (int d = 1; d < SPLIT; d <<= 1) {
+        st.count += __shfl_xor(st.count, d, 64);
+        st.hash += (unsigned long long)__shfl_xor((long long)st.hash, d, 64);
+    }
+}
+
 // floor(x / cellSize) clamped to [0, S-1] (systemHandling.cpp:39-41; the
 // reference leaves the x/cellSize == S case undefined).
 __device__ inline int cell_1d(double x, double cell, int S)

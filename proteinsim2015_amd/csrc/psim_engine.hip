@@ -186,6 +186,32 @@ void launch_forces_mask(psim_ctx *c)
     }
 }
 
+// Neighbour-set statistics with the launch shape launch_step_mask uses.
+void launch_stats(psim_ctx *c, NbrStatsOut so)
+{
+    if (c->dense) {
+        const int per = (c->ncell + 7) / 8;
+        k_stats_dense<<<8 * per, kDenseThreads, 0, c->stream>>>(c->geo, c->pot, c->prm.radius,
+                                                                c->pos, c->id, c->cstart, so,
+                                                                per);
+    } else if (c->brick) {
+        const int per = (c->bg.nbricks + 7) / 8;
+        if (c->brick_g == 1)
+            k_stats_brick<1><<<8 * per, kBrickThreads, 0, c->stream>>>(
+                c->geo, c->pot, c->prm.radius, c->bg, per, c->pos, c->id, c->cstart, so);
+        else
+            k_stats_brick<kSmallSplit><<<8 * per, kBrickThreads, 0, c->stream>>>(
+                c->geo, c->pot, c->prm.radius, c->bg, per, c->pos, c->id, c->cstart, so);
+    } else if (c->n < kSmallSystem) {
+        k_stats<64, kSmallSplit>
+            <<<xcd_grid(blocks_for((long long)c->n * kSmallSplit, 64)), 64, 0, c->stream>>>(
+                c->geo, c->pot, c->prm.radius, c->pos, c->id, c->cstart, so);
+    } else {
+        k_stats<kBlock, 1><<<xcd_grid(blocks_for(c->n, kBlock)), kBlock, 0, c->stream>>>(
+            c->geo, c->pot, c->prm.radius, c->pos, c->id, c->cstart, so);
+    }
+}
+
 template <int MASK>
 void launch_table_mask(psim_ctx *c)
 {
@@ -387,15 +413,8 @@ int psim_gpu_create(const psim_params *params, psim_ctx **out)
     if (!params || !out) return PSIM_E_ARG;
     *out = nullptr;
     const psim_params &P = *params;
-    if (P.n_particles <= 0 || P.n_particles >= (1 << kIdxBits)) return PSIM_E_ARG;
-    // the 27-cell stencil repeats cells for S < 3 (double counting in the
-    // reference, LJPotential.cpp:154-169)
-    if (P.cells_per_dim < 3 || P.cells_per_dim > 1290) return PSIM_E_UNSUPPORTED;
-    if (P.kernel_path < 0 || P.kernel_path > 3) return PSIM_E_ARG;
-    if (!(P.box > 0) || !(P.cell_size > 0) || !(P.cutoff > 0) || !(P.dt > 0) ||
-        !(P.mass > 0) || P.force_mask <= 0 || P.force_mask > 7)
-        return PSIM_E_ARG;
-    if ((P.force_mask & PSIM_FORCE_LJ) && !(P.debye_length > 0)) return PSIM_E_ARG;
+    const int chk = check_params(P);
+    if (chk) return chk;
     psim_ctx *c = new (std::nothrow) psim_ctx();
     if (!c) return PSIM_E_NOMEM;
     c->prm = P;
@@ -403,20 +422,11 @@ int psim_gpu_create(const psim_params *params, psim_ctx **out)
     c->n = P.n_particles;
     c->ncell = c->geo.ncell;
     c->nscan = (c->ncell + kScanTile - 1) / kScanTile;
-    // kernel path: the cell-per-workgroup kernel needs S >= 4 (shifted image
-    // = minimum image) and pays off once cells hold a few particles; below
-    // that, one thread per particle (bricks of cells per workgroup, path 3,
-    // give the same bits but measured slower at every sparse density)
-    const double occupancy = (double)c->n / c->ncell;
-    if (P.kernel_path == 1 || P.kernel_path == 3)
-        c->dense = false;
-    else if (P.kernel_path == 2)
-        c->dense = c->geo.S >= 4;
-    else
-        c->dense = c->geo.S >= 4 && occupancy >= kDenseMinOccupancy;
-    c->brick = P.kernel_path == 3;
-    c->brick_g = c->n < kSmallSystem ? kSmallSplit : 1;
-    c->bg = brick_shape(c->geo.S, occupancy, kBrickTarget * kBrickThreads / c->brick_g);
+    const PathChoice path = choose_path(P, c->geo);
+    c->dense = path.dense;
+    c->brick = path.brick;
+    c->brick_g = path.brick_g;
+    c->bg = path.bg;
     const size_t n3 = 3 * (size_t)c->n;
     int rc = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = PSIM_E_HIP;
@@ -479,7 +489,8 @@ void psim_gpu_destroy(psim_ctx *c)
     void *bufs[] = {c->pos, c->pos0, c->frc, c->mem, c->npos, c->npos0, c->nfrc,
                     c->nmem, c->vel, c->fout, c->id, c->nid, c->key, c->arank,
                     c->perm, c->count, c->cstart, c->bsum, c->err, c->noise,
-                    c->tsorted, c->thi, c->tcse, c->pbc, c->track0, c->hg.key,
+                    c->tsorted, c->thi, c->tcse, c->st_count, c->st_hash, c->pbc,
+                    c->track0, c->hg.key,
                     c->hg.rank, c->hg.perm, c->hg.count, c->hg.cstart, c->hg.bsum,
                     c->work, c->iwork};
     for (void *b : bufs)
@@ -643,6 +654,32 @@ int psim_gpu_force_table(psim_ctx *c, const double *sorted4, const int *hash_ind
     int code = 0;
     rc = read_error(c, &code, nullptr, nullptr);
     return rc ? rc : code;
+}
+
+int psim_gpu_pair_stats(psim_ctx *c, int which, int *count_by_orig,
+                        unsigned long long *hash_by_orig)
+{
+    if (!c || !c->have_state || !count_by_orig || !hash_by_orig) return PSIM_E_ARG;
+    if (which != PSIM_PAIRS_STEP && which != PSIM_PAIRS_ADJOINT) return PSIM_E_ARG;
+    const int n = c->n;
+    int rc = 0;
+    if (!c->st_count) {
+        if (!rc) rc = dalloc(&c->st_count, n);
+        if (!rc) rc = dalloc(&c->st_hash, n);
+        if (rc) return rc;
+    }
+    const NbrStatsOut so{c->st_count, c->st_hash};
+    if (which == PSIM_PAIRS_STEP)
+        launch_stats(c, so);
+    else
+        psim::eng::launch_adj_stats(c, so);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(count_by_orig, c->st_count, sizeof(int) * n, hipMemcpyDeviceToHost,
+                      c->stream));
+    CK(hipMemcpyAsync(hash_by_orig, c->st_hash, sizeof(unsigned long long) * n,
+                      hipMemcpyDeviceToHost, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 int psim_gpu_get_cells(psim_ctx *c, int *key_by_orig, int *cell_start, int *orig_by_slot)

@@ -110,6 +110,60 @@ __global__ __launch_bounds__(kBrickThreads) void k_ens_step_brick(
                                      epi);
 }
 
+// Neighbour-set statistics of the two ensemble pair kernels (their launch
+// shapes and acceptance code; see NbrStats).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_ens_stats(const RepGeo *__restrict__ reps,
+                                                     const int *__restrict__ p0, int nrep,
+                                                     int ntot, const double *pos, const int *id,
+                                                     const int *cstart, NbrStatsOut so)
+{
+    __shared__ unsigned hits[kHitCap * BLOCK];
+    const int s = xcd_block() * BLOCK + threadIdx.x;
+    if (s >= ntot) return;
+    const int r = replica_of(p0, nrep, id[s]);
+    const Geo g = reps[r].g;
+    const Pot p = reps[r].p;
+    const double radius = reps[r].radius;
+    const int N = ntot;
+    const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    SoAPos src{pos, N, radius};
+    StartTab tab{cstart};
+    Orig orig{id, 1, 0};
+    NbrStats st;
+    if (g.exact_mi)
+        gather_force<F_CAL, true, BLOCK>(g, p, src, tab, s, xi, yi, zi, radius,
+                                         hits + threadIdx.x, fx, fy, fz, nullptr, orig, 0, &st);
+    else
+        gather_force<F_CAL, false, BLOCK>(g, p, src, tab, s, xi, yi, zi, radius,
+                                          hits + threadIdx.x, fx, fy, fz, nullptr, orig, 0, &st);
+    so.count[id[s]] = st.count;
+    so.hash[id[s]] = st.hash;
+}
+
+__global__ __launch_bounds__(kBrickThreads) void k_ens_stats_brick(
+    const RepGeo *__restrict__ reps, const int *__restrict__ item0, int nrep, int nitems,
+    int per_xcd, const double *pos, const int *id, const int *cstart, NbrStatsOut so)
+{
+    __shared__ BrickShared sh;
+    const int it = xcd_cell(blockIdx.x, per_xcd);
+    if (it >= nitems) return;
+    const int r = replica_of(item0, nrep, it);
+    const Geo &g = reps[r].g;
+    const Pot &p = reps[r].p;
+    const double radius = reps[r].radius;
+    const BrickGeo &bg = reps[r].bg;
+    const int b = it - item0[r];
+    auto epi = [](int, double, double, double) {};
+    if (g.exact_mi)
+        brick_forces<F_CAL, true, 1, NbrStats>(g, p, radius, pos, cstart, id, nullptr, bg, b, sh,
+                                               epi, &so);
+    else
+        brick_forces<F_CAL, false, 1, NbrStats>(g, p, radius, pos, cstart, id, nullptr, bg, b,
+                                                sh, epi, &so);
+}
+
 // Initial hash of every particle into its replica's cells.
 __global__ void k_ens_hash(const RepGeo *__restrict__ reps, const int *__restrict__ p0,
                            int nrep, int ntot, const double *pos, const int *id, int *key,
@@ -165,6 +219,8 @@ struct psim_ens {
     int *count = nullptr, *cstart = nullptr, *bsum = nullptr;
     DevErr *err = nullptr;
     StepCtl *ctl = nullptr;
+    int *st_count = nullptr;                 // neighbour-set statistics
+    unsigned long long *st_hash = nullptr;
     bool have_state = false;
     int graph_steps = 0;
     hipGraph_t graph = nullptr;
@@ -431,7 +487,8 @@ void psim_ens_destroy(psim_ens *e)
     drop_graph(e);
     void *bufs[] = {e->pos, e->pos0, e->frc, e->mem, e->npos, e->npos0, e->nfrc, e->nmem,
                     e->vel, e->id, e->nid, e->key, e->arank, e->perm, e->count, e->cstart,
-                    e->bsum, e->err, e->ctl, e->reps, e->p0, e->item0};
+                    e->bsum, e->err, e->ctl, e->reps, e->p0, e->item0, e->st_count,
+                    e->st_hash};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -566,6 +623,41 @@ int psim_ens_last_error(psim_ens *e, int *code, int *i, int *j)
     if (code) *code = h.code;
     if (i) *i = h.i;
     if (j) *j = h.j;
+    return 0;
+}
+
+int psim_ens_pair_stats(psim_ens *e, int *count_by_orig, unsigned long long *hash_by_orig)
+{
+    if (!e || !e->have_state || !count_by_orig || !hash_by_orig) return PSIM_E_ARG;
+    const int n = e->ntot;
+    int rc = 0;
+    if (!e->st_count) {
+        if (!rc) rc = ealloc(&e->st_count, (size_t)n);
+        if (!rc) rc = ealloc(&e->st_hash, (size_t)n);
+        if (rc) return rc;
+    }
+    const NbrStatsOut so{e->st_count, e->st_hash};
+    if (e->brick) {
+        const int per = (e->nitems + 7) / 8;
+        k_ens_stats_brick<<<8 * per, kBrickThreads, 0, e->stream>>>(
+            e->reps, e->item0, e->nrep, e->nitems, per, e->pos, e->id, e->cstart, so);
+    } else {
+#define PSIM_ENS_STATS(BL)                                                                 \
+    k_ens_stats<BL><<<xcd_grid(eblocks(n, BL)), BL, 0, e->stream>>>(e->reps, e->p0, e->nrep, n, \
+                                                                    e->pos, e->id, e->cstart, so)
+        switch (e->block) {
+        case 64: PSIM_ENS_STATS(64); break;
+        case 128: PSIM_ENS_STATS(128); break;
+        default: PSIM_ENS_STATS(256); break;
+        }
+#undef PSIM_ENS_STATS
+    }
+    ECK(hipGetLastError());
+    ECK(hipMemcpyAsync(count_by_orig, e->st_count, sizeof(int) * n, hipMemcpyDeviceToHost,
+                       e->stream));
+    ECK(hipMemcpyAsync(hash_by_orig, e->st_hash, sizeof(unsigned long long) * n,
+                       hipMemcpyDeviceToHost, e->stream));
+    ECK(hipStreamSynchronize(e->stream));
     return 0;
 }
 

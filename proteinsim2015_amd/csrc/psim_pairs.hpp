@@ -125,12 +125,12 @@ __device__ inline double shifted(double xi, int k, double L)
 
 // Drain one lane's hit list: pair forces for entries [0, nh).  The list is a
 // column of stride BLOCK in the block's LDS array.
-template <int MASK, bool EXACT, int BLOCK = kBlock, class Src>
+template <int MASK, bool EXACT, int BLOCK = kBlock, class Src, class ST = NoStats>
 __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
                            double xi, double yi, double zi, double ri,
                            const unsigned *hits, int nh, double &fx,
                            double &fy, double &fz, DevErr *err,
-                           const Orig &orig)
+                           const Orig &orig, ST *st = nullptr)
 {
     for (int k = 0; k < nh; ++k) {
         unsigned e = hits[k * BLOCK];
@@ -160,6 +160,10 @@ __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
             dy = min_image_uv(yj - yi, g.L, g.half_uv);
             dz = min_image_uv(zj - zi, g.L, g.half_uv);
         }
+        if constexpr (ST::kOn) {
+            st->add(orig(j));
+            continue;
+        }
         double ov = p.ov_factor * size;
         if (r2 < ov * ov * (1.0 + p.ov_band)) {
             double r2x = pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc);
@@ -176,13 +180,13 @@ __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
 
 // Candidates of one neighbour cell h (image code `code`) appended to the
 // lane's hit list.
-template <int MASK, bool EXACT, int BLOCK, class Src>
+template <int MASK, bool EXACT, int BLOCK, class Src, class ST>
 __device__ __forceinline__ void cull_cell(const Geo &g, const Pot &p, const Src &src, int s,
                                           double xi, double yi, double zi, double xs,
                                           double ys, double zs, int b, int e, unsigned code,
                                           unsigned *hits, int &nh, double ri, double &fx,
                                           double &fy, double &fz, DevErr *err,
-                                          const Orig &orig)
+                                          const Orig &orig, ST *st)
 {
     for (int j = b; j < e; ++j) {
         bool hit;
@@ -203,7 +207,7 @@ __device__ __forceinline__ void cull_cell(const Geo &g, const Pot &p, const Src 
         }
         if (__any(nh == kHitCap)) {
             drain_hits<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, ri, hits, nh, fx, fy, fz,
-                                           err, orig);
+                                           err, orig, st);
             nh = 0;
         }
     }
@@ -213,13 +217,15 @@ __device__ __forceinline__ void cull_cell(const Geo &g, const Pot &p, const Src 
 // hits points at this lane's column of the block's LDS hit list.  With
 // SPLIT > 1, SPLIT consecutive lanes share the particle and lane `sub` walks
 // the x-runs sub, sub + SPLIT, ... of the 9 (its partial force; the caller
-// sums the SPLIT partials).
-template <int MASK, bool EXACT, int BLOCK = kBlock, int SPLIT = 1, class Src, class Tab>
+// sums the SPLIT partials).  With st (NbrStats), the accepted pairs are
+// recorded there instead of evaluated.
+template <int MASK, bool EXACT, int BLOCK = kBlock, int SPLIT = 1, class Src, class Tab,
+          class ST = NoStats>
 __device__ void gather_force(const Geo &g, const Pot &p, const Src &src,
                              const Tab &tab, int s, double xi, double yi,
                              double zi, double ri, unsigned *hits, double &fx,
                              double &fy, double &fz, DevErr *err,
-                             const Orig &orig, int sub = 0)
+                             const Orig &orig, int sub = 0, ST *st = nullptr)
 {
     const int S = g.S;
     const int cx = cell_1d(xi, g.cell, S);
@@ -240,11 +246,11 @@ __device__ void gather_force(const Geo &g, const Pot &p, const Src &src,
             cull_cell<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, shifted(xi, kx, g.L), ys, zs,
                                           sg ? xr.b1 : xr.b0, sg ? xr.e1 : xr.e0,
                                           (unsigned)(kx + 3 * ky + 9 * kz) << kIdxBits, hits,
-                                          nh, ri, fx, fy, fz, err, orig);
+                                          nh, ri, fx, fy, fz, err, orig, st);
         }
     }
     drain_hits<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, ri, hits, nh, fx, fy, fz, err,
-                                   orig);
+                                   orig, st);
 }
 
 }  // namespace psim

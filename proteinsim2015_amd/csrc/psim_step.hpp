@@ -4,6 +4,7 @@
 
 #include "psim_brick.hpp"
 #include "psim_dense.hpp"
+#include "psim_gpu.h"
 #include "psim_pairs.hpp"
 
 namespace psim {
@@ -23,6 +24,48 @@ constexpr int kSmallSplit = PSIM_SMALL_SPLIT;   // lanes per particle below kSma
 #define PSIM_BRICK_TARGET 0.8
 #endif
 constexpr double kBrickTarget = PSIM_BRICK_TARGET;
+
+// Checks of a psim_params shared by every entry point that takes one.
+inline int check_params(const psim_params &P)
+{
+    if (P.n_particles <= 0 || P.n_particles >= (1 << kIdxBits)) return PSIM_E_ARG;
+    // the 27-cell stencil repeats cells for S < 3 (double counting in the
+    // reference, LJPotential.cpp:154-169)
+    if (P.cells_per_dim < 3 || P.cells_per_dim > 1290) return PSIM_E_UNSUPPORTED;
+    if (P.kernel_path < 0 || P.kernel_path > 3) return PSIM_E_ARG;
+    if (!(P.box > 0) || !(P.cell_size > 0) || !(P.cutoff > 0) || !(P.dt > 0) ||
+        !(P.mass > 0) || P.force_mask <= 0 || P.force_mask > 7)
+        return PSIM_E_ARG;
+    if ((P.force_mask & PSIM_FORCE_LJ) && !(P.debye_length > 0)) return PSIM_E_ARG;
+    return 0;
+}
+
+// Pair-kernel path of a system: the cell-per-workgroup kernel needs S >= 4
+// (shifted image = minimum image) and pays off once cells hold a few
+// particles; below that, one thread per particle (bricks of cells per
+// workgroup, path 3, give the same bits but measured slower at every sparse
+// density).
+struct PathChoice {
+    bool dense;
+    bool brick;
+    int brick_g;   // lanes per particle of the brick step kernel
+    BrickGeo bg;
+};
+inline PathChoice choose_path(const psim_params &P, const Geo &g)
+{
+    PathChoice c{};
+    const double occupancy = (double)g.n / g.ncell;
+    if (P.kernel_path == 1 || P.kernel_path == 3)
+        c.dense = false;
+    else if (P.kernel_path == 2)
+        c.dense = g.S >= 4;
+    else
+        c.dense = g.S >= 4 && occupancy >= kDenseMinOccupancy;
+    c.brick = P.kernel_path == 3;
+    c.brick_g = g.n < kSmallSystem ? kSmallSplit : 1;
+    c.bg = brick_shape(g.S, occupancy, kBrickTarget * kBrickThreads / c.brick_g);
+    return c;
+}
 
 // Step control kept in device memory so that a captured sequence of steps
 // can be replayed: the step kernels read it, the reorder kernel advances it.
@@ -248,8 +291,25 @@ __global__ __launch_bounds__(256) void k_integrate(Geo g, GB gb, StepArgs a)
     integrate_one<false>(g, gb, a, s, a.nfrc[s], a.nfrc[N + s], a.nfrc[2 * N + s]);
 }
 
-// Forces only at the current state, written by original index ([3][N]).
-template <int MASK>
+// The force on the particle of original index o, by original index: [3][N]
+// (SoA) or, with AOS, N x 3.
+template <bool AOS>
+__device__ __forceinline__ void store_force(double *fout, int N, int o, double fx, double fy,
+                                            double fz)
+{
+    if (AOS) {
+        fout[3 * (size_t)o] = fx;
+        fout[3 * (size_t)o + 1] = fy;
+        fout[3 * (size_t)o + 2] = fz;
+    } else {
+        fout[o] = fx;
+        fout[N + o] = fy;
+        fout[2 * N + o] = fz;
+    }
+}
+
+// Forces only at the current state, written by original index.
+template <int MASK, bool AOS = false>
 __global__ __launch_bounds__(kBlock) void k_forces(Geo g, Pot p, double radius,
                                                    const double *pos, const int *id,
                                                    const int *cstart, double *fout,
@@ -270,14 +330,11 @@ __global__ __launch_bounds__(kBlock) void k_forces(Geo g, Pot p, double radius,
     else
         gather_force<MASK, false>(g, p, src, tab, s, xi, yi, zi, radius,
                                   hits + threadIdx.x, fx, fy, fz, err, orig);
-    const int o = id[s];
-    fout[o] = fx;
-    fout[N + o] = fy;
-    fout[2 * N + o] = fz;
+    store_force<AOS>(fout, N, id[s], fx, fy, fz);
 }
 
 // Forces only, brick per workgroup (same result as k_forces).
-template <int MASK>
+template <int MASK, bool AOS = false>
 __global__ __launch_bounds__(kBrickThreads) void k_forces_brick(Geo g, Pot p, double radius,
                                                                 BrickGeo bg, int per_xcd,
                                                                 const double *pos,
@@ -290,10 +347,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_forces_brick(Geo g, Pot p, do
     if (b >= bg.nbricks) return;
     const int N = g.n;
     auto epi = [&](int s, double fx, double fy, double fz) {
-        const int o = id[s];
-        fout[o] = fx;
-        fout[N + o] = fy;
-        fout[2 * N + o] = fz;
+        store_force<AOS>(fout, N, id[s], fx, fy, fz);
     };
     if (g.exact_mi)
         brick_forces<MASK, true, 1>(g, p, radius, pos, cstart, id, err, bg, b, sh, epi);
@@ -301,7 +355,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_forces_brick(Geo g, Pot p, do
         brick_forces<MASK, false, 1>(g, p, radius, pos, cstart, id, err, bg, b, sh, epi);
 }
 
-template <int MASK, int LJN>
+template <int MASK, int LJN, bool AOS = false>
 __global__ __launch_bounds__(kDenseThreads)
 __attribute__((amdgpu_waves_per_eu(dense_waves(MASK)))) void k_forces_dense(
     Geo g, Pot p, double radius, const double *pos, const int *id, const int *cstart,
@@ -314,11 +368,75 @@ __attribute__((amdgpu_waves_per_eu(dense_waves(MASK)))) void k_forces_dense(
     const int N = g.n;
     dense_cell_forces<MASK, kDenseThreads, LJN>(g, p, radius, pos, cstart, id, err, c, sh,
                             [=](int i, double Fx, double Fy, double Fz) {
-                                const int o = id[home0 + i];
-                                fout[o] = Fx;
-                                fout[N + o] = Fy;
-                                fout[2 * N + o] = Fz;
+                                store_force<AOS>(fout, N, id[home0 + i], Fx, Fy, Fz);
                             });
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour-set statistics of the pair kernels: the launch shapes and the
+// acceptance code of the step kernels (k_step with BLOCK / SPLIT,
+// k_step_dense, k_step_brick with G), each accepted pair recorded in an
+// NbrStats instead of evaluated; counts and partner hashes by original index.
+// ---------------------------------------------------------------------------
+template <int BLOCK, int SPLIT>
+__global__ __launch_bounds__(BLOCK) void k_stats(Geo g, Pot p, double radius, const double *pos,
+                                                 const int *id, const int *cstart,
+                                                 NbrStatsOut so)
+{
+    __shared__ unsigned hits[kHitCap * BLOCK];
+    const int t = xcd_block() * BLOCK + threadIdx.x;
+    const int s = t / SPLIT, sub = t % SPLIT;
+    if (s >= g.n) return;   // whole lane groups
+    const int N = g.n;
+    const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    SoAPos src{pos, N, radius};
+    StartTab tab{cstart};
+    Orig orig{id, 1, 0};
+    NbrStats st;
+    if (g.exact_mi)
+        gather_force<F_CAL, true, BLOCK, SPLIT>(g, p, src, tab, s, xi, yi, zi, radius,
+                                                hits + threadIdx.x, fx, fy, fz, nullptr, orig,
+                                                sub, &st);
+    else
+        gather_force<F_CAL, false, BLOCK, SPLIT>(g, p, src, tab, s, xi, yi, zi, radius,
+                                                 hits + threadIdx.x, fx, fy, fz, nullptr, orig,
+                                                 sub, &st);
+    split_stats<SPLIT>(st);
+    if (sub == 0) {
+        so.count[id[s]] = st.count;
+        so.hash[id[s]] = st.hash;
+    }
+}
+
+__global__ __launch_bounds__(kDenseThreads)
+__attribute__((amdgpu_waves_per_eu(dense_waves(F_CAL)))) void k_stats_dense(
+    Geo g, Pot p, double radius, const double *pos, const int *id, const int *cstart,
+    NbrStatsOut so, int per_xcd)
+{
+    __shared__ DenseShared sh;
+    const int c = xcd_cell(blockIdx.x, per_xcd);
+    if (c >= g.ncell) return;
+    dense_cell_forces<F_CAL, kDenseThreads, 0, NbrStats>(
+        g, p, radius, pos, cstart, id, nullptr, c, sh, [](int, double, double, double) {}, &so);
+}
+
+template <int G>
+__global__ __launch_bounds__(kBrickThreads) void k_stats_brick(Geo g, Pot p, double radius,
+                                                               BrickGeo bg, int per_xcd,
+                                                               const double *pos, const int *id,
+                                                               const int *cstart, NbrStatsOut so)
+{
+    __shared__ BrickShared sh;
+    const int b = xcd_cell(blockIdx.x, per_xcd);
+    if (b >= bg.nbricks) return;
+    auto epi = [](int, double, double, double) {};
+    if (g.exact_mi)
+        brick_forces<F_CAL, true, G, NbrStats>(g, p, radius, pos, cstart, id, nullptr, bg, b, sh,
+                                               epi, &so);
+    else
+        brick_forces<F_CAL, false, G, NbrStats>(g, p, radius, pos, cstart, id, nullptr, bg, b,
+                                                sh, epi, &so);
 }
 
 // Forces from the reference's host arrays (IForce::getAcceleration over the

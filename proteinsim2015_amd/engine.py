@@ -25,7 +25,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import FORCE_AO, FORCE_CAL, FORCE_LJ, PsimParams, check
+from ._lib import FORCE_AO, FORCE_CAL, FORCE_LJ, PAIRS_ADJOINT, PAIRS_STEP, PsimParams, check
 
 _NUM = re.compile(r"^\s*[+-]?(\d+\.?\d*([eE][+-]?\d+)?|\.\d+([eE][+-]?\d+)?)")
 _INT = re.compile(r"^\s*[+-]?\d+")
@@ -409,6 +409,19 @@ class GpuSystem:
             check(st, "psim_gpu_force_table")
         return st, out.reshape(self.n, 3)
 
+    def pair_stats(self, adjoint: bool = False):
+        """Neighbour sets as the pair kernel sees them: per particle (original
+        index) the accepted partner count and the partner-hash sum (uint64);
+        adjoint=True uses the trajectory adjoint's pair visitor instead of the
+        step kernel's."""
+        cnt = np.empty(self.n, np.int32)
+        hs = np.empty(self.n, np.uint64)
+        check(self.lib.psim_gpu_pair_stats(self._ctx, PAIRS_ADJOINT if adjoint else PAIRS_STEP,
+                                           cnt.ctypes.data_as(C.POINTER(C.c_int)),
+                                           hs.ctypes.data_as(C.POINTER(C.c_ulonglong))),
+              "psim_gpu_pair_stats")
+        return cnt, hs
+
     def cells(self):
         n, nc = self.n, self.geometry.ncell
         key = np.empty(n, np.int32)
@@ -672,6 +685,16 @@ class Ensemble:
         check(self.lib.psim_ens_last_error(self._ctx, C.byref(code), C.byref(i), C.byref(j)),
               "psim_ens_last_error")
         return code.value, i.value, j.value
+
+    def pair_stats(self):
+        """Per-particle accepted partner count and partner-hash sum of every
+        replica (concatenated original index), from the current pair kernel."""
+        cnt = np.empty(self.n, np.int32)
+        hs = np.empty(self.n, np.uint64)
+        check(self.lib.psim_ens_pair_stats(self._ctx, cnt.ctypes.data_as(C.POINTER(C.c_int)),
+                                           hs.ctypes.data_as(C.POINTER(C.c_ulonglong))),
+              "psim_ens_pair_stats")
+        return cnt, hs
 
     def autotune(self, trial_steps: int = 5):
         """Time the pair-kernel candidates (thread per particle with workgroup

@@ -103,3 +103,59 @@ def test_ensemble_reports_overlap():
         code, i, j = ens.last_error()
     assert code == 7704
     assert {i, j} == {s1.start, s1.start + 1}
+
+
+def test_full_512_replica_ensemble(oracle):
+    """The batched workload at its own size: 512 seeded AO replicas (N 500-20000,
+    phi 0.05-0.35, 2-8 kT, q 0.05-0.2; 5.25 M particles), autotuned and
+    graph-replayed.  Every replica: positions inside its box, its net force
+    ~ 0 and its neighbour sets equal to the oracle's after the run; eight
+    seeded replicas (the smallest, the largest, the densest and five more)
+    follow the oracle's trajectory."""
+    from proteinsim2015_amd import bench_ensemble as be
+    reps = be.make_replicas(512)
+    pos_l, vel_l = be.initial_states(reps)
+    pos, vel = np.concatenate(pos_l), np.concatenate(vel_l)
+    integ = ps.BrownianIntegrator(kT=1.0, mass=1.0, gamma=1.0, dt=5e-4, vel_freq=1000,
+                                  seed=90210)
+    print(f"ensemble 512: {len(pos)} particles", flush=True)
+    with ps.Ensemble(reps, integ) as ens:
+        ens.set_state(pos, vel)
+        kernel, tile, _ = ens.autotune(1)      # 6 x (1 + 1) = 12 steps
+        ens.set_graph(4)
+        ens.step(8)
+        ens.sync()
+        assert ens.last_error()[0] == 0
+        out = ens.get_state()
+        cnt, hs = ens.pair_stats()
+        offsets = ens.offsets
+    nsteps = 12 + 8
+    print(f"ensemble 512: kernel {kernel} tile {tile}", flush=True)
+    phis = np.array([r.geometry.volume_fraction for r in reps])
+    sizes = np.array([r.geometry.n for r in reps])
+    sample = {int(np.argmin(sizes)), int(np.argmax(sizes)), int(np.argmax(phis))}
+    rng = np.random.default_rng(77)
+    while len(sample) < 8:
+        sample.add(int(rng.integers(len(reps))))
+    c = oracle.gb_coeffs(1.0, 1.0, 1.0, 5e-4)
+    for r, rep in enumerate(reps):
+        g = rep.geometry
+        sl = slice(offsets[r], offsets[r + 1])
+        p = out["pos"][sl]
+        assert p.min() >= 0.0 and p.max() < g.box, f"replica {r}"
+        f = out["frc"][sl]
+        assert np.abs(f.sum(axis=0)).max() <= 1e-9 * max(1.0, np.abs(f).sum()), f"replica {r}"
+        fp = ForceParams.make(PO_AO, rep.forces.cutoff, kT=1.0,
+                              ao_well_depth=rep.forces.well_depth)
+        oc, oh = oracle.pair_stats(fp, p, g.cells_per_dim, g.cell_size, g.box,
+                                   id0=int(offsets[r]))
+        assert np.array_equal(cnt[sl], oc) and np.array_equal(hs[sl], oh), f"replica {r}"
+        if r in sample:
+            p0 = np.where(np.abs(pos[sl]) > g.box / 2,
+                          np.where(pos[sl] < 0, -g.box, g.box), 0.0)
+            st = dict(pos=pos[sl].copy(), pos0=p0, vel=vel[sl].copy(),
+                      frc=np.zeros((g.n, 3)), frc0=np.zeros((g.n, 3)), memx=np.zeros((g.n, 3)))
+            code, _ = oracle.run(fp, c, st, nsteps, g.box, g.cells_per_dim, g.cell_size,
+                                 vel_freq=1000, seed=90210, id0=int(offsets[r]))
+            assert code == 0
+            assert np.abs(p - st["pos"]).max() <= 1e-10, f"replica {r}"
