@@ -185,6 +185,22 @@ enum { PSIM_PAIRS_STEP = 0, PSIM_PAIRS_ADJOINT = 1 };
 int psim_gpu_pair_stats(psim_ctx *ctx, int which, int *count_by_orig,
                         unsigned long long *hash_by_orig);
 
+/* Forces at caller-held positions with NO device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  params as for psim_gpu_create; d_pos and d_force are
+ * DEVICE arrays of n x 3 doubles by original index; stream is a hipStream_t
+ * (NULL: the null stream).  The cell build's working arrays live in a
+ * static device array of PSIM_NOALLOC_SCRATCH_BYTES per GPU (held until
+ * the process ends), so n and S^3 are bounded: about 40 n + 8 S^3 bytes
+ * must fit (else PSIM_E_NOMEM and a message on stderr).  Calls that overlap
+ * on one GPU race on that array.  Returns after the forces are written: 0,
+ * 7704 or 7705 as psim_gpu_forces, or PSIM_E_*.  The forces equal
+ * psim_gpu_forces' on a context with the same params and positions. */
+#define PSIM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int psim_gpu_forces_noalloc(const psim_params *params, const double *d_pos, double *d_force,
+                            void *stream);
+
 /* Cell data of the current positions for inspection: key_by_orig[n]
  * (cell hash per original index), cell_start[S^3+1] (slot offsets),
  * orig_by_slot[n] (original index stored at each slot). */

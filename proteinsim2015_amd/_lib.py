@@ -74,7 +74,7 @@ EXPORTS = (
     "psim_gpu_get_params", "psim_gpu_get_tracker", "psim_gpu_set_tracker",
     "psim_write_recovery", "psim_read_recovery", "psim_write_xyz", "psim_gpu_save",
     "psim_gpu_load", "psim_write_sys_config", "psim_gpu_kernel_info",
-    "psim_gpu_pair_stats", "psim_ens_pair_stats",
+    "psim_gpu_pair_stats", "psim_ens_pair_stats", "psim_gpu_forces_noalloc",
 )
 
 _vp = C.c_void_p
@@ -134,6 +134,7 @@ _SIGS = {
     "psim_gpu_kernel_info": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "psim_gpu_pair_stats": ([_vp, C.c_int, _ip, _up], C.c_int),
     "psim_ens_pair_stats": ([_vp, _ip, _up], C.c_int),
+    "psim_gpu_forces_noalloc": ([C.POINTER(PsimParams), _vp, _vp, _vp], C.c_int),
     "psim_gpu_get_tracker": ([_vp, _dp, _ip], C.c_int),
     "psim_gpu_set_tracker": ([_vp, _dp, _ip], C.c_int),
     "psim_write_recovery": ([C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double],

@@ -59,7 +59,7 @@ def build_product(verbose: bool = False) -> str:
     os.makedirs(tmp, exist_ok=True)
     jobs = []
     for src in ("psim_engine.hip", "psim_ensemble.hip", "psim_analysis.hip",
-                "psim_adjoint.hip", "psim_state.hip"):
+                "psim_adjoint.hip", "psim_state.hip", "psim_noalloc.hip"):
         obj = os.path.join(tmp, src + ".o")
         jobs.append((obj, [hipcc, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c",
                            os.path.join(CSRC, src), "-o", obj]))
