@@ -8,13 +8,17 @@
 //   per-particle getAcceleration loop    defaultForceManager.cpp:55-65
 // and, with mode "batch", the plugin's psim_force_batch entry instead.
 //
-//   psim_host_demo <plugin.so> <settings.cfg> <pos.f64> <force.f64> [batch|each]
+//   psim_host_demo <plugin.so> <settings.cfg> <pos.f64> <force.f64> [batch|each] [evals]
 //
 // pos.f64 holds nParticles x 3 doubles (original order); force.f64 receives
-// particleForce (nParticles x 3 doubles, original order).
+// particleForce (nParticles x 3 doubles, original order).  With evals > 1
+// the evaluation is repeated at currentTime = k * timeStep, k = 0..evals-1
+// (one force evaluation per step, as system::run makes), and the mean wall
+// time of the evaluations after the first is printed.
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -60,6 +64,7 @@ int main(int argc, char **argv)
         return 2;
     }
     const std::string mode = argc > 5 ? argv[5] : "each";
+    const int evals = argc > 6 ? std::max(1, std::atoi(argv[6])) : 1;
     PSim::config cfg;
     std::ifstream in(argv[2]);
     for (std::string line; std::getline(in, line);) {
@@ -126,13 +131,16 @@ int main(int argc, char **argv)
     }
     PSim::IForce *force = factory(&cfg);
     std::vector<double> out(3 * (size_t)n, 0.0);
+    batch_fn *batch = nullptr;
+    std::vector<int> hi, ce;
     if (mode == "batch") {
-        batch_fn *batch = (batch_fn *)dlsym(lib, "psim_force_batch");
+        batch = (batch_fn *)dlsym(lib, "psim_force_batch");
         if (!batch) {
             std::fprintf(stderr, "plugin has no psim_force_batch\n");
             return 100;
         }
-        std::vector<int> hi(2 * (size_t)n), ce(2 * cse.size());
+        hi.resize(2 * (size_t)n);
+        ce.resize(2 * cse.size());
         for (int i = 0; i < n; ++i) {
             hi[2 * i] = std::get<0>(phi[i]);
             hi[2 * i + 1] = std::get<1>(phi[i]);
@@ -141,16 +149,27 @@ int main(int argc, char **argv)
             ce[2 * h] = std::get<0>(cse[h]);
             ce[2 * h + 1] = std::get<1>(cse[h]);
         }
-        int rc = batch(force, sorted.data(), out.data(), hi.data(), ce.data(), &st);
-        if (rc) {
-            std::fprintf(stderr, "psim_force_batch returned %d\n", rc);
-            return rc > 0 ? rc : 3;
-        }
-    } else {
-#pragma omp parallel for
-        for (int i = 0; i < n; ++i)
-            force->getAcceleration(i, sorted.data(), out.data(), &phi, &cse, &st);
     }
+    double later = 0.0;
+    for (int k = 0; k < evals; ++k) {
+        st.currentTime = k * st.dTime;
+        const auto t0 = std::chrono::steady_clock::now();
+        if (batch) {
+            int rc = batch(force, sorted.data(), out.data(), hi.data(), ce.data(), &st);
+            if (rc) {
+                std::fprintf(stderr, "psim_force_batch returned %d\n", rc);
+                return rc > 0 ? rc : 3;
+            }
+        } else {
+#pragma omp parallel for
+            for (int i = 0; i < n; ++i)
+                force->getAcceleration(i, sorted.data(), out.data(), &phi, &cse, &st);
+        }
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                              .count();
+        if (k > 0) later += dt;
+    }
+    if (evals > 1) std::printf("seconds per evaluation: %.6e\n", later / (evals - 1));
     std::ofstream of(argv[4], std::ios::binary);
     of.write(reinterpret_cast<const char *>(out.data()), sizeof(double) * out.size());
     std::printf("%s: %d particles, L=%d, cell=%d, S=%d, mode=%s\n", force->getName().c_str(), n,

@@ -67,3 +67,33 @@ def test_demo_host_per_particle_and_batch(oracle, tmp_path, mode):
     fp = force_params(mask, cfg)
     denom = pair_abs_sum(g["pos"], int(g["box"]), fp.cutoff, pair_fmag(fp))
     assert force_rel_err(f, g["force"], denom).max() <= 1e-6
+
+
+def test_per_particle_calls_cost_one_evaluation_per_step(tmp_path):
+    """N = 262,144 AO through the unchanged per-particle contract (one
+    getAcceleration call per sorted index, from OpenMP threads): each step
+    costs one GPU evaluation plus O(1) per call, in the same range as the
+    one-call batch entry, and gives the same forces bit for bit."""
+    import proteinsim2015_amd as ps
+    from cases import PHI_P_C2
+    n, conc, scale = 262144, 0.15, 48          # reference geometry: box 96, cell 2
+    settings = {"nParticles": n, "conc": conc, "scale": scale, "radius": 0.5,
+                "cutOff": 1.1, "kT": 1.0, "wellDepth": PHI_P_C2}
+    (tmp_path / "settings.cfg").write_text("".join(f"{k} = {v!r}\n" if isinstance(v, float)
+                                                   else f"{k} = {v}\n"
+                                                   for k, v in settings.items()))
+    ps.rsa_positions(n, 96.0, 1234).astype(np.float64).tofile(tmp_path / "pos.f64")
+    secs, forces = {}, {}
+    for mode in ("batch", "each"):
+        print(f"plugin 262144: mode={mode}", flush=True)
+        r = subprocess.run([os.path.join(PLUGIN_DIR, "psim_host_demo"), plugin_path("AO"),
+                            str(tmp_path / "settings.cfg"), str(tmp_path / "pos.f64"),
+                            str(tmp_path / f"f_{mode}.f64"), mode, "6"],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        line = [x for x in r.stdout.splitlines() if x.startswith("seconds per evaluation")][0]
+        secs[mode] = float(line.split(":")[1])
+        forces[mode] = np.fromfile(tmp_path / f"f_{mode}.f64")
+    print(f"plugin 262144: {secs}", flush=True)
+    assert np.array_equal(forces["each"], forces["batch"])
+    assert secs["each"] <= 10 * secs["batch"] + 0.05
