@@ -273,6 +273,34 @@ int psim_ens_pair_stats(psim_ens *ens, int *count_by_orig, unsigned long long *h
 int psim_ens_autotune(psim_ens *ens, int trial_steps, int *kernel, int *tile,
                       float *ms_per_step);
 
+/* ---- G&B integrator over a host's particle arrays -------------------------
+ * The GPU side of the IIntegrator adaptor (include/psim/IIntegrator.h):
+ * brownianIntegrator's constructor (brownianIntegrator.cpp:27-99) and
+ * nextSystem (:181-356) for a host that owns its particles.  params gives
+ * n_particles, kT, gamma, dt, vel_freq and seed (the reference's int seed);
+ * mass is per particle in psim_int_next.  noise PSIM_NOISE_REFERENCE draws
+ * the reference's own per-particle fastRandom streams (threadSeed =
+ * seed * (i + 1), fastRandom.cpp:40-124) on the device; PSIM_NOISE_PHILOX
+ * draws Philox4x32-10 normals keyed by (seed, call count, particle index).
+ * Device memory: about 340 bytes per particle. */
+enum { PSIM_NOISE_REFERENCE = 0, PSIM_NOISE_PHILOX = 1 };
+typedef struct psim_integrator psim_integrator;
+int psim_int_create(const psim_params *params, int noise, psim_integrator **out);
+void psim_int_destroy(psim_integrator *integ);
+/* One IIntegrator::nextSystem call (IIntegrator.h:44): current_time == 0
+ * takes firstStep (:191-223), else normalStep (:225-296) with velocityStep
+ * (:298-356) when velFreq == 0 or the call counter reaches velFreq.  Host
+ * arrays in the host's particle order: pos, pos0 (updated as
+ * particle::setPos leaves them, particle.cpp:61-99) and vel (updated on
+ * velocity steps) are n x 3 in/out; frc, frc0 (n x 3) and mass (n) are
+ * read.  Returns 0, PSIM_ERR_BOUNDS (7703, *err_index = the particle) or
+ * PSIM_E_UNSUPPORTED when a reference stream re-seeds on every draw and
+ * cannot produce a deviate (a negative threadSeed: the reference loops
+ * forever there). */
+int psim_int_next(psim_integrator *integ, double current_time, int box, double *pos,
+                  double *pos0, double *vel, const double *frc, const double *frc0,
+                  const double *mass, int *err_index);
+
 /* ---- analysis of the current state --------------------------------------- */
 
 /* Replace the force-module parameters (force_mask, lj_num, cutoff,

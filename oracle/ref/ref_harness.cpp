@@ -71,6 +71,9 @@ void rebuild(particle **parts, const systemState &st, Cells &c)
     c.start_end.assign(ncell, tuple<int, int>(0xffffffff, 0xffffffff));
     c.sorted.assign(4 * (size_t)n, 0.0);
     int sq = scale * scale;
+    // the reference runs the hash and the reorder as OpenMP loops
+    // (systemHandling.cpp:35, 55)
+#pragma omp parallel for
     for (int i = 0; i < n; i++) {
         int cx = floor(parts[i]->getX() / st.cellSize);
         int cy = floor(parts[i]->getY() / st.cellSize);
@@ -79,6 +82,7 @@ void rebuild(particle **parts, const systemState &st, Cells &c)
         get<1>(c.hash_index[i]) = i;
     }
     __gnu_parallel::sort(c.hash_index.begin(), c.hash_index.end(), util::sortParticleTuple);
+#pragma omp parallel for
     for (int i = 0; i < n; i++) {
         int h = get<0>(c.hash_index[i]);
         if (i == 0) get<0>(c.start_end[h]) = 0;
@@ -148,6 +152,22 @@ systemState make_state(int n, int box, int cell, int scale, double dt, int seed)
     return st;
 }
 
+// An integrator from a library that exports getIntegrator(config*) (a host
+// that loads its integrator by name, as runSim.cpp:35-61 loads forces).
+IIntegrator *load_integrator(const char *so_path, config *cfg)
+{
+    void *lib = dlopen(so_path, RTLD_LAZY);
+    if (!lib) {
+        std::fprintf(stderr, "ref_harness: dlopen(%s) failed: %s\n", so_path, dlerror());
+        return nullptr;
+    }
+    dlerror();
+    typedef IIntegrator *create_Integrator(config *);
+    create_Integrator *factory = (create_Integrator *)dlsym(lib, "getIntegrator");
+    if (dlerror()) return nullptr;
+    return factory(cfg);
+}
+
 }  // namespace
 
 extern "C" {
@@ -199,20 +219,26 @@ int ref_forces(const char *so_path, const char *cfg_path, int n,
     return 0;
 }
 
-// nsteps of the system::run loop (system.cpp:350-374) without the contact
+// nsteps of the system::run loop (system.cpp:187-211) without the contact
 // pass and analysis output.  Trajectory arrays hold (nsteps+1) x 3n doubles;
 // slot 0 is the state after construction (tr_pos NULL: no trajectory).
-// loop_seconds receives the wall time of the step loop alone.
+// loop_seconds receives the wall time of the step loop alone, from the end
+// of step `skip` on (0: the whole loop).  integ_so NULL: the reference's
+// brownianIntegrator; else the integrator that library's
+// getIntegrator(config*) returns.
 int ref_run(const char *so_path, const char *cfg_path, int n,
             const double *pos_in, const double *vel_in, double radius,
             double mass, int box, int cell, int scale, double dt, int nthreads,
             int nsteps, double *tr_pos, double *tr_pos0, double *tr_vel,
-            double *tr_frc, double *tr_frc0, double *loop_seconds)
+            double *tr_frc, double *tr_frc0, double *loop_seconds, const char *integ_so,
+            int skip)
 {
     config cfg(cfg_path);
     IForce *f = load_force(so_path, &cfg);
     if (!f) return -1;
-    brownianIntegrator *integ = new brownianIntegrator(&cfg);
+    IIntegrator *integ = integ_so ? load_integrator(integ_so, &cfg)
+                                  : new brownianIntegrator(&cfg);
+    if (!integ) return -2;
     particle **p = make_particles(n, pos_in, vel_in, radius, mass, box);
     systemState st = make_state(n, box, cell, scale, dt, 0);
     defaultForceManager *mgr = new defaultForceManager();
@@ -226,13 +252,19 @@ int ref_run(const char *so_path, const char *cfg_path, int n,
     for (int s = 0; s < nsteps; s++) {
         rebuild(p, st, c);
         mgr->getAcceleration(c.sorted.data(), force.data(), &c.hash_index, &c.start_end, &st);
-        for (int i = 0; i < n; i++) p[i]->setForce(&force[3 * (size_t)i]);   // pushParticleForce
+        // pushParticleForce (systemHandling.cpp:87-92)
+#pragma omp parallel for
+        for (int i = 0; i < n; i++) p[i]->setForce(&force[3 * (size_t)i]);
         integ->nextSystem(p, &st);
         st.currentTime += st.dTime;
         size_t o = (size_t)(s + 1) * stride;
         double r0 = omp_get_wtime();
         if (tr_pos) record(p, n, tr_pos + o, tr_pos0 + o, tr_vel + o, tr_frc + o, tr_frc0 + o);
         rec += omp_get_wtime() - r0;
+        if (s + 1 == skip) {
+            t0 = omp_get_wtime();
+            rec = 0.0;
+        }
     }
     if (loop_seconds) *loop_seconds = omp_get_wtime() - t0 - rec;
     for (int i = 0; i < n; i++) delete p[i];

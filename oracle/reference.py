@@ -50,7 +50,8 @@ class Reference:
         L.ref_run.argtypes = [C.c_char_p, C.c_char_p, C.c_int, _dp, _dp, C.c_double,
                               C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
                               C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
-                              C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
+                              C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_char_p,
+                              C.c_int]
         L.ref_run.restype = C.c_int
         L.ref_run_sampled.argtypes = [C.c_char_p, C.c_char_p, C.c_int, _dp, _dp, C.c_double,
                                       C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
@@ -95,8 +96,11 @@ class Reference:
         return f.reshape(-1, 3), perm, cs, ce, sorted4.reshape(-1, 4)
 
     def run(self, plugin, cfg: dict, pos, vel, radius, mass, box, cell, scale, dt,
-            nsteps, nthreads=8, record=True):
-        """Trajectory dict (record=True) or the step-loop seconds (record=False)."""
+            nsteps, nthreads=8, record=True, integrator=None, skip=0):
+        """Trajectory dict (record=True) or the step-loop seconds (record=False)
+        of the steps after the first ``skip``.  integrator: path of a library
+        exporting getIntegrator(config*) to use instead of the reference's
+        brownianIntegrator."""
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
         n = len(pos)
@@ -109,7 +113,9 @@ class Reference:
             st = self.lib.ref_run(self.plugin(plugin), path.encode(), n, pos.ravel(),
                                   vel.ravel(), float(radius), float(mass), int(box),
                                   int(cell), int(scale), float(dt), int(nthreads),
-                                  int(nsteps), *(ptr(k) for k in keys), C.byref(secs))
+                                  int(nsteps), *(ptr(k) for k in keys), C.byref(secs),
+                                  None if integrator is None
+                                  else os.path.abspath(integrator).encode(), int(skip))
         finally:
             os.unlink(path)
         if st:

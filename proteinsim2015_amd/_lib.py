@@ -75,6 +75,7 @@ EXPORTS = (
     "psim_write_recovery", "psim_read_recovery", "psim_write_xyz", "psim_gpu_save",
     "psim_gpu_load", "psim_write_sys_config", "psim_gpu_kernel_info",
     "psim_gpu_pair_stats", "psim_ens_pair_stats", "psim_gpu_forces_noalloc",
+    "psim_int_create", "psim_int_destroy", "psim_int_next",
 )
 
 _vp = C.c_void_p
@@ -82,6 +83,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _up = C.POINTER(C.c_ulonglong)
 PAIRS_STEP, PAIRS_ADJOINT = 0, 1
+NOISE_REFERENCE, NOISE_PHILOX = 0, 1
 
 _SIGS = {
     "psim_gpu_abi_version": ([], C.c_int),
@@ -135,6 +137,9 @@ _SIGS = {
     "psim_gpu_pair_stats": ([_vp, C.c_int, _ip, _up], C.c_int),
     "psim_ens_pair_stats": ([_vp, _ip, _up], C.c_int),
     "psim_gpu_forces_noalloc": ([C.POINTER(PsimParams), _vp, _vp, _vp], C.c_int),
+    "psim_int_create": ([C.POINTER(PsimParams), C.c_int, C.POINTER(_vp)], C.c_int),
+    "psim_int_destroy": ([_vp], None),
+    "psim_int_next": ([_vp, C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip], C.c_int),
     "psim_gpu_get_tracker": ([_vp, _dp, _ip], C.c_int),
     "psim_gpu_set_tracker": ([_vp, _dp, _ip], C.c_int),
     "psim_write_recovery": ([C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double],

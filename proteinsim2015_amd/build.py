@@ -4,7 +4,8 @@
 
 Products (git-ignored, shipped with the tree to the GPU host):
   proteinsim2015_amd/libpsim_gpu.so        HIP kernels (gfx950) + C ABI
-  proteinsim2015_amd/plugins/*.so          PSim::IForce plugin adaptors
+  proteinsim2015_amd/plugins/*.so          PSim::IForce plugin adaptors and the
+                                           PSim::IIntegrator adaptor (GBIntegrator.so)
   proteinsim2015_amd/cli/psim_sim          the simulation program (settings.cfg
                                            in, trial directory out)
 Test infrastructure built alongside (never loaded by the product):
@@ -59,7 +60,8 @@ def build_product(verbose: bool = False) -> str:
     os.makedirs(tmp, exist_ok=True)
     jobs = []
     for src in ("psim_engine.hip", "psim_ensemble.hip", "psim_analysis.hip",
-                "psim_adjoint.hip", "psim_state.hip", "psim_noalloc.hip"):
+                "psim_adjoint.hip", "psim_state.hip", "psim_noalloc.hip",
+                "psim_integrator.hip"):
         obj = os.path.join(tmp, src + ".o")
         jobs.append((obj, [hipcc, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c",
                            os.path.join(CSRC, src), "-o", obj]))
@@ -92,6 +94,11 @@ def build_plugins(verbose: bool = False) -> None:
               f"-DPSIM_PLUGIN_MASK={mask}", src, "-o", out + ".tmp", "-L", PKG, "-lpsim_gpu",
               "-Wl,-rpath,$ORIGIN/.."], verbose)
         os.replace(out + ".tmp", out)
+    integ = os.path.join(PLUGIN_DIR, "GBIntegrator.so")
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-fopenmp", "-Wall", "-I", INCLUDE,
+          os.path.join(PLUGIN_DIR, "psim_integrator_plugin.cpp"), "-o", integ + ".tmp", "-L",
+          PKG, "-lpsim_gpu", "-Wl,-rpath,$ORIGIN/.."], verbose)
+    os.replace(integ + ".tmp", integ)
     demo = os.path.join(PLUGIN_DIR, "psim_host_demo")
     _run(["g++", "-std=c++17", "-O2", "-fopenmp", "-Wall", "-I", INCLUDE,
           os.path.join(PLUGIN_DIR, "psim_host_demo.cpp"), "-o", demo + ".tmp", "-ldl"], verbose)

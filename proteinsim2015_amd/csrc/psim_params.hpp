@@ -12,52 +12,18 @@
 
 namespace psim {
 
-inline void make_constants(const psim_params &P, Geo &g, Pot &p, GB &b)
+// The G&B coefficients of brownianIntegrator.cpp:53-81, 111-175 (host FP64,
+// glibc libm, the reference's formulas and operation order): setupHigh,
+// the series of setupLow when gamma < 0.05, setupZero when gamma == 0, then
+// the noise widths and their correlation.
+struct GBCoeffs {
+    double coeff0, coeff1, coeff2, coeff3;
+    double sig1, sig2, corr, dev;
+    double gn, goy2, goy3, hn;
+};
+
+inline GBCoeffs gb_coefficients(double kT, double gamma, double dt)
 {
-    g.n = P.n_particles;
-    g.S = P.cells_per_dim;
-    g.S2 = g.S * g.S;
-    g.ncell = g.S2 * g.S;
-    g.L = P.box;
-    g.half_pbc = P.box / 2.0;
-    if (P.half_uv > 0)
-        g.half_uv = P.half_uv;
-    else if (std::floor(P.box) == P.box)
-        g.half_uv = (double)((long long)P.box / 2);   // int L / 2 (utilities.cpp:124)
-    else
-        g.half_uv = P.box / 2.0;
-    g.cell = P.cell_size;
-    g.exact_mi = (g.S < 4) ? 1 : 0;
-    g.c0 = 0;
-
-    p.mask = P.force_mask;
-    p.lj_num = P.lj_num;
-    const double rc = P.cutoff;
-    p.rc2 = rc * rc;
-    // guard band: the fast image distance differs from pbcDist's by a few
-    // ulps of L on each component
-    double band = 1e-12 * std::fmax(1.0, P.box / rc);
-    p.rc2_hi = p.rc2 * (1.0 + band);
-    p.rc2_lo = p.rc2 * (1.0 - band);
-    p.ov_factor = 0.8;
-    p.ov_band = band;
-    p.lj_neg_kt_eps = -P.kT * P.lj_well_depth;
-    p.lj_four_n = 4.0 * P.lj_num;
-    p.debye_len = P.debye_length;
-    p.debye_inv = 1.0 / P.debye_length;
-    p.yuk_a = P.yukawa_strength;
-    p.lj_att = p.lj_neg_kt_eps * p.lj_four_n;
-    p.lj_rep = p.lj_neg_kt_eps * p.yuk_a;
-    // AOPotential.cpp:50-55
-    double q = (rc / (rc - 1.0));
-    double a1 = -P.kT * P.ao_well_depth * q * q * q;
-    double a2 = -3.0 / (2.0 * rc);
-    double a3 = 1.0 / (2.0 * rc * rc * rc);
-    p.ao_c1 = -a1 * a2;
-    p.ao_c2 = -3.0 * a1 * a3;
-
-    // brownianIntegrator.cpp:53-81, 111-175 (host FP64, glibc libm)
-    const double kT = P.kT, gamma = P.gamma, dt = P.dt;
     double y = gamma * dt;
     double coeff0, coeff1, coeff2, coeff3, sig1, sig2, gn, goy2, goy3, hn;
     {
@@ -102,6 +68,58 @@ inline void make_constants(const psim_params &P, Geo &g, Pot &p, GB &b)
     sig2 = std::sqrt(-kT * sig2 / gamma2);
     double corr = (kT / (gamma2)) * (gn / (sig1 * sig2));
     double dev = std::sqrt(1.0 - (corr * corr));
+    return GBCoeffs{coeff0, coeff1, coeff2, coeff3, sig1, sig2, corr, dev, gn, goy2, goy3, hn};
+}
+
+inline void make_constants(const psim_params &P, Geo &g, Pot &p, GB &b)
+{
+    g.n = P.n_particles;
+    g.S = P.cells_per_dim;
+    g.S2 = g.S * g.S;
+    g.ncell = g.S2 * g.S;
+    g.L = P.box;
+    g.half_pbc = P.box / 2.0;
+    if (P.half_uv > 0)
+        g.half_uv = P.half_uv;
+    else if (std::floor(P.box) == P.box)
+        g.half_uv = (double)((long long)P.box / 2);   // int L / 2 (utilities.cpp:124)
+    else
+        g.half_uv = P.box / 2.0;
+    g.cell = P.cell_size;
+    g.exact_mi = (g.S < 4) ? 1 : 0;
+    g.c0 = 0;
+
+    p.mask = P.force_mask;
+    p.lj_num = P.lj_num;
+    const double rc = P.cutoff;
+    p.rc2 = rc * rc;
+    // guard band: the fast image distance differs from pbcDist's by a few
+    // ulps of L on each component
+    double band = 1e-12 * std::fmax(1.0, P.box / rc);
+    p.rc2_hi = p.rc2 * (1.0 + band);
+    p.rc2_lo = p.rc2 * (1.0 - band);
+    p.ov_factor = 0.8;
+    p.ov_band = band;
+    p.lj_neg_kt_eps = -P.kT * P.lj_well_depth;
+    p.lj_four_n = 4.0 * P.lj_num;
+    p.debye_len = P.debye_length;
+    p.debye_inv = 1.0 / P.debye_length;
+    p.yuk_a = P.yukawa_strength;
+    p.lj_att = p.lj_neg_kt_eps * p.lj_four_n;
+    p.lj_rep = p.lj_neg_kt_eps * p.yuk_a;
+    // AOPotential.cpp:50-55
+    double q = (rc / (rc - 1.0));
+    double a1 = -P.kT * P.ao_well_depth * q * q * q;
+    double a2 = -3.0 / (2.0 * rc);
+    double a3 = 1.0 / (2.0 * rc * rc * rc);
+    p.ao_c1 = -a1 * a2;
+    p.ao_c2 = -3.0 * a1 * a3;
+
+    const double dt = P.dt;
+    const GBCoeffs k = gb_coefficients(P.kT, P.gamma, P.dt);
+    const double coeff0 = k.coeff0, coeff1 = k.coeff1, coeff2 = k.coeff2, coeff3 = k.coeff3;
+    const double sig1 = k.sig1, sig2 = k.sig2, corr = k.corr, dev = k.dev;
+    const double goy2 = k.goy2, goy3 = k.goy3, hn = k.hn;
     b.coeff0 = coeff0;
     b.coeff1 = coeff1;
     b.coeff3 = coeff3;
