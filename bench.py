@@ -9,10 +9,17 @@ G&B integrator dt = 1e-3, gamma = 1, kT = 1, in-kernel Philox noise.
 One step = forces + integration + cell rebuild (the system::run loop body).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--no-small] [--no-ensemble] [--no-adjoint]
 
 Prints one JSON line.  The state (positions, velocities, noise generator)
 is resident in HBM before the timed region; the timed region is K steps
-bracketed by stream synchronisations.
+bracketed by stream synchronisations.  Beside the headline the line
+carries: the whole-step roofline (algorithmic bytes per particle-step
+against the HBM peak) with the pair kernel's VALU issue fraction, the
+pair-work counters, the N = 4096 Calibration + LJ and N = 262,144 AO
+workloads, the 512-replica batched ensemble against one system per
+launch, the trajectory adjoint's step time, and the reference's own CPU
+code timed on the host cores (median of 3 runs).
 """
 from __future__ import annotations
 
@@ -45,11 +52,11 @@ PAIR_KERNEL_BYTES = 48.0
 INTEGRATE_KERNEL_BYTES = 208.0
 
 
-def step_bytes(ncell: int) -> float:
+def step_bytes(ncell: int, n: int = N) -> float:
     """Algorithmic bytes per particle-step of the whole step, in-kernel noise:
     FP64 integrator state r/w 192 B, cell build 16 B, sorted position copy
     48 B, cell-start table 8 B per cell."""
-    return 192.0 + 16.0 + 48.0 + 8.0 * ncell / N
+    return 192.0 + 16.0 + 48.0 + 8.0 * ncell / n
 
 
 def parse():
@@ -58,7 +65,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=25)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the N=4096 Calibration+LJ and N=262,144 AO workloads")
     ap.add_argument("--path", default="auto", choices=["auto", "sparse", "dense"])
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 512-replica batched-ensemble measurement")
@@ -108,11 +118,24 @@ def stdout_to_stderr():
         os.close(saved)
 
 
-def cpu_baseline(pos_exact, box_exact, steps):
+def cpu_model() -> str:
+    """The host CPU's model name (first entry of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pos_exact, box_exact, steps, runs=3):
     """The reference's own code (oracle/_ref, compiled from its sources) on
     the host cores, in the reference's integer geometry for this workload
-    (boxSize 153, cellSize 3, scale 51); the oracle restatement if the
-    reference build is absent."""
+    (boxSize 153, cellSize 3, scale 51): ``steps`` steady-state steps after
+    one untimed first step, median of ``runs`` runs; the oracle restatement
+    if the reference build is absent."""
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = max(1, min(threads, os.cpu_count() or 1))
     os.environ["OMP_NUM_THREADS"] = str(threads)
@@ -124,12 +147,13 @@ def cpu_baseline(pos_exact, box_exact, steps):
     cfg = {"cutOff": CUTOFF, "kT": 1.0, "wellDepth": 1.0, "yukawaStrength": YUK_A,
            "ljNum": 18, "debyeLength": DEBYE, "nParticles": N, "gamma": 1.0, "mass": 1.0,
            "timeStep": 0.001, "velFreq": 1000, "seed": 1}
-    sample = (f"N={N}, reference integer geometry L={box_i} cell={cell_i} S={scale_i}, "
-              f"{steps} steps incl. the first, OpenMP threads={threads}")
+    secs = []
     if reference_available():
         from oracle import Reference
-        secs = Reference().run("LJ", cfg, pos, vel, 0.5, 1.0, box_i, cell_i, scale_i, 0.001,
-                               steps, nthreads=threads, record=False)
+        ref = Reference()
+        for _ in range(runs):
+            secs.append(ref.run("LJ", cfg, pos, vel, 0.5, 1.0, box_i, cell_i, scale_i, 0.001,
+                                steps + 1, nthreads=threads, record=False, skip=1))
         kind = "reference"
     else:
         from oracle import ForceParams, load_oracle
@@ -137,15 +161,22 @@ def cpu_baseline(pos_exact, box_exact, steps):
         fp = ForceParams.make(2, CUTOFF, kT=1.0, lj_well_depth=1.0, yukawa_strength=YUK_A,
                               debye_length=DEBYE, lj_num=18)
         c = o.gb_coeffs(1.0, 1.0, 1.0, 1e-3)
-        st = dict(pos=pos, pos0=np.zeros((N, 3)), vel=vel, frc=np.zeros((N, 3)),
-                  frc0=np.zeros((N, 3)), memx=np.zeros((N, 3)))
-        t0 = time.perf_counter()
-        o.run(fp, c, st, steps, float(box_i), scale_i, float(cell_i), seed=1,
-              nthreads=threads)
-        secs = time.perf_counter() - t0
+        for _ in range(runs):
+            st = dict(pos=pos.copy(), pos0=np.zeros((N, 3)), vel=vel.copy(),
+                      frc=np.zeros((N, 3)), frc0=np.zeros((N, 3)), memx=np.zeros((N, 3)))
+            o.run(fp, c, st, 1, float(box_i), scale_i, float(cell_i), seed=1, nthreads=threads)
+            t0 = time.perf_counter()
+            o.run(fp, c, st, steps, float(box_i), scale_i, float(cell_i), seed=1,
+                  nthreads=threads)
+            secs.append(time.perf_counter() - t0)
         kind = "port"
-    return {"value": N * steps / secs, "unit": "particle-steps/s", "cores": threads,
-            "kind": kind, "sample": sample, "seconds": secs}
+    med = float(np.median(secs))
+    sample = (f"N={N}, reference integer geometry L={box_i} cell={cell_i} S={scale_i}: "
+              f"{steps} steady-state steps after one untimed step, median of {runs} runs "
+              f"({', '.join(f'{x:.2f}' for x in secs)} s), OpenMP threads={threads}")
+    return {"value": N * steps / med, "unit": "particle-steps/s", "cores": threads,
+            "kind": kind, "sample": sample, "seconds": med, "cpu_model": cpu_model(),
+            "runs_seconds": secs}
 
 
 def cpu_adjoint_baseline(steps=20):
@@ -183,35 +214,89 @@ def cpu_adjoint_baseline(steps=20):
                       f"OpenMP threads={threads}", "seconds": secs}
 
 
-def traffic_from_profiles():
-    """HBM bytes per k_step launch measured by rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE) and calibrated against a copy kernel; null when absent."""
+def c3_profile() -> dict:
+    """The committed rocprofv3 figures for this workload (HBM bytes from
+    FETCH_SIZE / WRITE_SIZE scaled by a copy kernel, VALU instructions of
+    the pair kernel); empty when absent."""
     path = os.path.join(ROOT, "profiles", "k_step_c3_traffic.json")
-    if not os.path.exists(path):
-        return None
     try:
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            return json.load(f)
     except (OSError, ValueError):
-        return None
+        return {}
 
 
-def valu_from_profiles(pair_ms):
-    """VALU lane-op rate of the pair kernel from a committed rocprofv3 --pmc
-    run (SQ_INSTS_VALU x 64 lanes per launch / measured launch time)."""
-    path = os.path.join(ROOT, "profiles", "k_step_c3_traffic.json")
-    try:
-        with open(path) as f:
-            d = json.load(f)
-        insts = d["valu_insts_per_launch"]
-    except (OSError, ValueError, KeyError):
+def valu_from_profiles(prof, pair_ms):
+    """VALU issue of the pair kernel: SQ_INSTS_VALU (committed rocprofv3
+    --pmc run) over the live launch time, against one wave64 instruction per
+    SIMD every 4 cycles (the FP64 FMA rate, 39.3 T lane-ops/s)."""
+    insts = prof.get("valu_insts_per_launch")
+    if not insts or not pair_ms:
         return None
     rate = insts * 64 / (pair_ms * 1e-3)
-    return {"achieved_lane_ops_per_s": rate, "peak": VALU_PEAK_LANE_OPS,
+    return {"kernel": prof.get("kernel"), "insts_per_launch": insts,
+            "insts_per_particle_step_lanes": insts * 64 / N,
+            "achieved_lane_ops_per_s": rate, "peak": VALU_PEAK_LANE_OPS,
             "frac": rate / VALU_PEAK_LANE_OPS,
-            "note": "SQ_INSTS_VALU x 64 counts issued lanes incl. masked ones; the peak "
-                    "issues one wave64 instruction per SIMD every 4 cycles (the FP64 FMA "
-                    "rate), so a frac near 1 means VALU-issue bound"}
+            "note": "SQ_INSTS_VALU x 64 counts issued lanes incl. masked ones; a frac "
+                    "near 1 means VALU-issue bound"}
+
+
+def small_leg(name: str, steps: int, warmup: int = 10) -> dict:
+    """One of the smaller plain-engine workloads, timed like the headline."""
+    import proteinsim2015_amd as ps
+    if name == "c1":
+        n, phi = 4096, 0.20
+        geom = ps.SystemGeometry.exact(n, phi, 2.5)
+        mods = [ps.Calibration(cutoff=2.5),
+                ps.LennardJones(kT=1.0, well_depth=1.0, yukawa_strength=0.0, lj_num=6,
+                                cutoff=2.5, debye_length=0.5)]
+        dt = 1e-3
+        wl = (f"N=4096 Calibration + LJ 12-6 (eps 1, cutoff 2.5), phi=0.20, L={geom.box:.3f}, "
+              f"{geom.cells_per_dim}^3 cells, dt=1e-3, {steps} steps")
+    else:
+        n, phi = 262144, 0.15
+        geom = ps.SystemGeometry.exact(n, phi, 1.1)
+        mods = [ps.AOPotential(kT=1.0, well_depth=0.25, cutoff=1.1)]
+        dt = 5e-4
+        wl = (f"N=262,144 AO q=0.1 (phi_p 0.25, 4 kT at contact, cutoff 1.1) + r^-38 core, "
+              f"phi=0.15, L={geom.box:.3f}, {geom.cells_per_dim}^3 cells, dt=5e-4, "
+              f"{steps} steps")
+    integ = ps.BrownianIntegrator(kT=1.0, mass=1.0, gamma=1.0, dt=dt, vel_freq=1000, seed=90210)
+    pos = ps.rsa_positions(n, geom.box, 1234)
+    vel = np.random.default_rng(4321).standard_normal((n, 3))
+    with ps.GpuSystem(geom, mods, integ) as sysg:
+        sysg.set_state(pos, vel=vel)
+        sysg.step_device_noise(warmup, 0)
+        sysg.sync()
+        t0 = time.perf_counter()
+        sysg.step_device_noise(steps, 0)
+        sysg.sync()
+        el = time.perf_counter() - t0
+        if sysg.last_error()[0]:
+            raise SystemExit(f"bench: physics error {sysg.last_error()[0]} in {name}")
+        # kernel split from a separate profiled pass (events per step would
+        # weigh on steps of a few microseconds)
+        sysg.set_profiling(True)
+        sysg.kernel_times(reset=True)
+        sysg.step_device_noise(min(steps, 200), 0)
+        kt = sysg.kernel_times(reset=True)
+        sysg.set_profiling(False)
+        cnt, _ = sysg.pair_stats()
+        info = sysg.kernel_info()
+    value = n * steps / el
+    b_alg = step_bytes(geom.ncell, n)
+    pair_ms = kt["force_ms"] / max(1, kt["force_launches"])
+    return {"workload": wl, "n_particles": n, "steps": steps, "kernel_path": info["path"],
+            "ms_per_step": el / steps * 1e3, "particle_steps_per_s": value,
+            "pair_kernel_ms": pair_ms,
+            "g_and_b_kernel_ms": kt["integrate_ms"] / max(1, kt["integrate_launches"])
+            if kt["integrate_launches"] else 0.0,
+            "cell_rebuild_ms": kt["build_ms"] / max(1, kt["build_launches"]),
+            "bytes_per_particle_step": b_alg,
+            "hbm_frac": b_alg * value / (HBM_PEAK_GBS * 1e9),
+            "candidates_per_particle_step": 27.0 * n / geom.ncell,
+            "in_range_pairs_per_particle_step": float(cnt.mean())}
 
 
 def main():
@@ -263,11 +348,24 @@ def main():
     build_ms = kt["build_ms"] / max(1, kt["build_launches"])
     pair_bytes = PAIR_KERNEL_BYTES if kt["integrate_launches"] else \
         PAIR_KERNEL_BYTES + INTEGRATE_KERNEL_BYTES - 48.0
-    achieved = pair_bytes * N / (pair_ms * 1e-3) / 1e9
-    wall_step_bytes = step_bytes(geom.ncell)
+    pair_gbs = pair_bytes * N / (pair_ms * 1e-3) / 1e9
+    b_alg = step_bytes(geom.ncell)
+    achieved = b_alg * value / 1e9
     print(f"bench: {args.steps} steps in {elapsed:.3f}s; pair kernel {pair_ms:.3f} ms, "
           f"G&B kernel {integ_ms:.3f} ms, cell rebuild {build_ms:.3f} ms", file=sys.stderr,
           flush=True)
+    cnt, _ = sysg.pair_stats()
+    in_range = float(cnt.mean())
+    info = sysg.kernel_info()
+    sysg.close()
+    prof = c3_profile()
+    small = {}
+    if not args.no_small:
+        for name, steps in (("c1", 1000), ("c2", 10000)):
+            small[name] = small_leg(name, steps)
+            print(f"bench: {name} {small[name]['particle_steps_per_s']:.3g} particle-steps/s "
+                  f"({small[name]['ms_per_step']:.4f} ms per step)", file=sys.stderr,
+                  flush=True)
     sysg.close()
     ensemble = None
     if not args.no_ensemble:
@@ -283,7 +381,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         with stdout_to_stderr():
-            cpu = cpu_baseline(pos, geom.box, args.cpu_steps)
+            cpu = cpu_baseline(pos, geom.box, args.cpu_steps, args.cpu_runs)
         if adjoint is not None:
             acpu = cpu_adjoint_baseline()
             adjoint["cpu_baseline"] = acpu
@@ -314,23 +412,39 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {
-            "bound": "hbm",
-            "kernel": "k_step_dense (27-cell LDS-staged pair forces)" if kt["integrate_launches"]
-                      else "k_step (27-cell pair forces + fused G&B)",
+            # the whole step against the HBM peak on the algorithmic bytes per
+            # particle-step; the step's dominant kernel is bound by VALU issue
+            "bound": "valu",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(),
-            "bytes_per_particle": pair_bytes,
-            "avg_launch_ms": pair_ms,
+            "traffic": prof.get("step_bytes_per_step"),
+            "bytes_per_particle_step": b_alg,
+            "traffic_note": "achieved = B_alg x particle-steps/s with B_alg = 192 + 16 + 48 + "
+                            "8 ncell/N bytes per particle-step; traffic = HBM bytes per step "
+                            "of the step's kernels from rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                            "scaled by a copy kernel (profiles/)",
+            "valu": valu_from_profiles(prof, pair_ms),
+            "pair_kernel": {
+                "kernel": f"k_step_dense<2,18> ({info['path']} path)" if kt["integrate_launches"]
+                          else f"k_step ({info['path']} path, fused G&B)",
+                "avg_launch_ms": pair_ms,
+                "bytes_per_particle": pair_bytes,
+                "achieved_gbs": pair_gbs,
+                "hbm_frac": pair_gbs / HBM_PEAK_GBS,
+                "traffic": prof.get("bytes_per_launch"),
+            },
             "g_and_b_kernel_ms": integ_ms,
             "cell_rebuild_ms": build_ms,
-            "whole_step_bytes_per_particle": wall_step_bytes,
-            "whole_step_achieved_gbs": wall_step_bytes * value / 1e9,
-            "whole_step_frac": wall_step_bytes * value / (HBM_PEAK_GBS * 1e9),
-            "valu": valu_from_profiles(pair_ms),
         },
+        "pair_work": {
+            "candidates_per_particle_step": 27.0 * N / geom.ncell,
+            "in_range_pairs_per_particle_step": in_range,
+            "note": "candidates = particles in the 27-cell stencil (incl. self); in-range = "
+                    "pairs the kernel's exact test accepts (psim_gpu_pair_stats)",
+        },
+        "other_workloads": small or None,
         "cpu_baseline": cpu,
         "batched_ensemble": ensemble,
         "adjoint": adjoint,
