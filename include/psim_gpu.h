@@ -14,13 +14,13 @@
  * Which reference interface each entry point replaces:
  *   psim_gpu_create        PSim::system::setSystemConstants + the force
  *                          plugin / integrator constructors
- *                          (system.cpp:190-230, LJPotential.cpp:29-62,
+ *                          (system.cpp:27-67, LJPotential.cpp:29-62,
  *                          AOPotential.cpp:33-59, Calibration.cpp:30-39,
  *                          brownianIntegrator.cpp:27-99)
  *   psim_gpu_set_state     particle setters + the constructor's first cell
- *                          build (particle.cpp:61-99, system.cpp:259-266)
+ *                          build (particle.cpp:61-99, system.cpp:96-103)
  *   psim_gpu_step          nsteps iterations of system::run
- *                          (system.cpp:350-374): forces
+ *                          (system.cpp:187-211): forces
  *                          (defaultForceManager.cpp:55-65 ->
  *                          IForce::getAcceleration, IForce.h:35),
  *                          pushParticleForce (systemHandling.cpp:87-92),

@@ -17,7 +17,7 @@
 #endif
 
 /* ------------------------------------------------------------------ */
-/* Geometry: system.cpp:219-227                                         */
+/* Geometry: system.cpp:50-64                                          */
 /* ------------------------------------------------------------------ */
 void po_geometry(int n, double conc, int scale, double radius,
                  int *box, int *cell_size, double *concentration)
@@ -25,8 +25,8 @@ void po_geometry(int n, double conc, int scale, double radius,
     /* vP = N * (4/3) * pi * r^3 with pi = atan(1)*4, evaluated left to right */
     double vP = n * (4.0 / 3.0) * atan(1.0) * 4.0 * radius * radius * radius;
     int b = (int)(cbrt(vP / conc));
-    int c = b / scale;           /* integer division, system.cpp:223 */
-    b = c * scale;               /* system.cpp:224 */
+    int c = b / scale;           /* integer division, system.cpp:60 */
+    b = c * scale;               /* system.cpp:61 */
     *box = b;
     *cell_size = c;
     *concentration = vP / pow((double)b, 3.0);
@@ -692,7 +692,7 @@ void po_philox_normals(unsigned long long seed, unsigned long long step,
 }
 
 /* ------------------------------------------------------------------ */
-/* Full step loop: system.cpp:350-374                                    */
+/* Full step loop: system.cpp:187-211                                    */
 /* ------------------------------------------------------------------ */
 int po_run(const po_force_params *fp, const po_gb_coeffs *c,
            int n, double box, double half_uv, int S, double cell_size,

@@ -69,7 +69,7 @@ typedef struct po_gb_coeffs {
     double goy2, goy3, hn, gn;
 } po_gb_coeffs;
 
-/* system.cpp:219-227 (reference integer geometry). */
+/* system.cpp:50-64 (reference integer geometry). */
 void po_geometry(int n, double conc, int scale, double radius,
                  int *box, int *cell_size, double *concentration);
 
@@ -146,7 +146,7 @@ void po_philox_normals(unsigned long long seed, unsigned long long step,
 void po_philox4x32_10(const unsigned int ctr[4], const unsigned int key[2],
                       unsigned int out[4]);
 
-/* Full steps in the order of system::run (system.cpp:350-374): build cells,
+/* Full steps in the order of system::run (system.cpp:187-211): build cells,
  * forces, push (frc0<-frc, frc<-F), integrate, time += dt.  State arrays are
  * AoS3 in original order.  noise: NULL -> Philox(seed, *step_index, id);
  * else [nsteps][6][n] by original index (first step reads the B slots).

@@ -19,7 +19,7 @@ constexpr unsigned kIdxMask = (1u << kIdxBits) - 1u;
 
 enum { F_CAL = 1, F_LJ = 2, F_AO = 4 };
 
-// Box and cell geometry (system.cpp:219-227; systemState.h:7-19).
+// Box and cell geometry (system.cpp:50-64; systemState.h:7-19).
 struct Geo {
     int n;          // particles in this system
     int S;          // cells per dimension (cellScale)

@@ -9,7 +9,7 @@
 //   vel[3][N] is kept by ORIGINAL index (read on the first step, written on
 //   velocity steps only).
 //
-// One step (the order of system::run, system.cpp:350-374):
+// One step (the order of system::run, system.cpp:187-211):
 //   k_step      fused: 27-cell forces F(x_n) (IForce::getAcceleration),
 //               push (frc0 <- frc), G&B integration (firstStep/normalStep/
 //               velocityStep), wrap (particle::setPos), new cell hash and
@@ -273,7 +273,7 @@ int run_steps(psim_ctx *c, int nsteps, const double *d_noise)
             }
         }
         c->step_index += 1;
-        c->time += c->prm.dt;   // system.cpp:373
+        c->time += c->prm.dt;   // system.cpp:210
     }
     return 0;
 }

@@ -179,7 +179,7 @@ __global__ void k_ens_hash(const RepGeo *__restrict__ reps, const int *__restric
 }
 
 // End of step: advance the clock and set the flags of the next step
-// (system.cpp:373; brownianIntegrator.cpp:183, 288-292).
+// (system.cpp:210; brownianIntegrator.cpp:183, 288-292).
 __global__ void k_ctl_advance(StepCtl *c)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {

@@ -70,7 +70,7 @@ inline PathChoice choose_path(const psim_params &P, const Geo &g)
 // Step control kept in device memory so that a captured sequence of steps
 // can be replayed: the step kernels read it, the reorder kernel advances it.
 // (brownianIntegrator.cpp:183 first-step test, :288-292 velocity counter,
-// system.cpp:373 time advance)
+// system.cpp:210 time advance)
 struct StepCtl {
     unsigned long long step;   // Philox counter of the current step
     double time;               // currentTime at the start of the step

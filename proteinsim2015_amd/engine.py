@@ -10,7 +10,7 @@ Vocabulary follows the reference:
   * ``BrownianIntegrator`` -- brownianIntegrator's parameters
                         (brownianIntegrator.cpp:27-99);
   * ``SystemGeometry`` -- systemState's box/cell geometry
-                        (system.cpp:190-227);
+                        (system.cpp:27-67);
   * ``GpuSystem``      -- one simulated system on the GPU: the state, the
                         force evaluation and the step loop of system::run.
 """
@@ -199,7 +199,7 @@ class SystemGeometry:
 
     @classmethod
     def from_config(cls, cfg: Config):
-        """system::setSystemConstants (system.cpp:190-227): integer box and
+        """system::setSystemConstants (system.cpp:27-67): integer box and
         cell, box = cellSize * scale."""
         n = cfg.get_param("nParticles", 1000)
         conc = cfg.get_param("conc", 0.01)

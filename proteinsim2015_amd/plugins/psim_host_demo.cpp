@@ -2,7 +2,7 @@
 //
 // Mirrors how the reference host finds and drives a force plugin:
 //   settings file (key = value)          config.cpp:27-69
-//   geometry from nParticles/conc/scale  system.cpp:219-227
+//   geometry from nParticles/conc/scale  system.cpp:50-64
 //   dlopen(plugin) + dlsym("getForce")   runSim.cpp:29-61 (exit 100 on failure)
 //   cell build: hash, sort, start/end    systemHandling.cpp:33-85
 //   per-particle getAcceleration loop    defaultForceManager.cpp:55-65

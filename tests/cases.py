@@ -3,7 +3,7 @@
 Each force case names the reference plugin it pins, the reference config
 keys handed to that plugin (the same keys the reference reads, see
 LJPotential.cpp:34-54, AOPotential.cpp:40-47, Calibration.cpp:36), and a
-reference-integer geometry (system.cpp:219-227).
+reference-integer geometry (system.cpp:50-64).
 """
 from __future__ import annotations
 
