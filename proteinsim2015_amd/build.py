@@ -39,6 +39,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wshadow", "-Wno-unused-result", "-Wno-unused-function",
 ]
+# tile / launch-shape overrides for sweeps, e.g. PSIM_HIP_DEFINES="PSIM_DENSE_WAVES=6"
+HIP_FLAGS += [f"-D{d}" for d in os.environ.get("PSIM_HIP_DEFINES", "").split()]
 
 
 def _hipcc() -> str:

@@ -341,6 +341,8 @@ int save_cp(psim_adjoint *A, int k)
     const size_t n3 = 3 * (size_t)c->n;
     double *d = A->cp_d + (size_t)k * cp_stride_d(c);
     int *ip = A->cp_i + (size_t)k * cp_stride_i(c);
+    int rc = eng::settle(c);   // checkpoints hold the state in the current slot order
+    if (rc) return rc;
     const double *src[5] = {c->pos, c->pos0, c->frc, c->mem, c->vel};
     CopySegs cs{};
     for (int q = 0; q < 5; ++q) add_seg(cs, src[q], d + q * n3, sizeof(double) * n3);
@@ -363,6 +365,7 @@ int restore_cp(psim_adjoint *A, int k)
     add_seg(cs, ip + c->n, c->cstart, sizeof(int) * (c->ncell + 1));
     int rc = copy_segs(c, cs);
     if (rc) return rc;
+    c->lagged = false;
     c->time = A->meta[k].time;
     c->vel_counter = A->meta[k].vel_counter;
     c->step_index = A->meta[k].step;

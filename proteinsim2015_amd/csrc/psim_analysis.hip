@@ -495,7 +495,9 @@ int psim_gpu_observables(psim_ctx *c, double contact_r, psim_observables *out,
     if (!c || !out || !c->have_state || !(contact_r >= 0.0)) return PSIM_E_ARG;
     const int n = c->n;
     const unsigned nb = blocks_for(n, kAnaBlock);
-    int rc = ensure_work(c, (size_t)nb * 4, (size_t)n);
+    int rc = psim::eng::settle(c);   // the mean displacement reads pos0
+    if (rc) return rc;
+    rc = ensure_work(c, (size_t)nb * 4, (size_t)n);
     if (rc) return rc;
     k_observables<<<nb, kAnaBlock, 0, c->stream>>>(c->geo, c->pos, c->pos0, c->vel, c->id,
                                                    c->cstart, c->track0, c->pbc, contact_r,

@@ -200,5 +200,52 @@ __global__ void k_move(int n, const int *__restrict__ perm, const int *__restric
 }
 
 
+// As k_move, but only the positions and the original ids are gathered into
+// the new order; prev[dst] records the slot each particle came from, so the
+// integrator reads the previous positions, forces and memory of the last
+// step where it wrote them (at prev[s]) instead of every array moving every
+// step.
+__global__ void k_move_lagged(int n, const int *__restrict__ perm, const int *__restrict__ key,
+                              const int *__restrict__ cstart, const double *__restrict__ npos,
+                              const int *__restrict__ id_in, double *__restrict__ pos,
+                              int *__restrict__ id_out, int *__restrict__ prev)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int s = perm[t];
+    const int c = key[s];
+    const int b = cstart[c], e = cstart[c + 1];
+    int rank = 0;
+    int u = b;
+    for (; u + 4 <= e; u += 4) {
+        const int p0 = perm[u], p1 = perm[u + 1], p2 = perm[u + 2], p3 = perm[u + 3];
+        rank += (p0 < s) + (p1 < s) + (p2 < s) + (p3 < s);
+    }
+    for (; u < e; ++u) rank += (perm[u] < s);
+    const int dst = b + rank;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pos[d * n + dst] = npos[d * n + s];
+    id_out[dst] = id_in[s];
+    prev[dst] = s;
+}
+
+// The lagged arrays (previous positions, last forces, memory, kept in the
+// order of the step that wrote them) gathered into the current slot order.
+__global__ void k_settle(int n, const int *__restrict__ prev, const double *__restrict__ pos0,
+                         const double *__restrict__ frc, const double *__restrict__ mem,
+                         double *__restrict__ pos0_out, double *__restrict__ frc_out,
+                         double *__restrict__ mem_out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int p = prev[s];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        pos0_out[d * n + s] = pos0[d * n + p];
+        frc_out[d * n + s] = frc[d * n + p];
+        mem_out[d * n + s] = mem[d * n + p];
+    }
+}
+
 }  // namespace
 }  // namespace psim

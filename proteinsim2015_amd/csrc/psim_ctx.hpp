@@ -11,6 +11,15 @@
 #include "psim_device.hpp"
 #include "psim_gpu.h"
 
+// The buffers and constants a captured step sequence bakes into its kernels.
+struct GraphKey {
+    const void *pos, *pos0, *frc, *mem, *npos, *npos0, *nfrc, *nmem, *id, *nid, *pbc;
+    psim::Geo geo;
+    psim::Pot pot;
+    psim::GB gb;
+    unsigned long long seed;
+};
+
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
@@ -29,6 +38,18 @@ struct psim_ctx {
     double *npos = nullptr, *npos0 = nullptr, *nfrc = nullptr, *nmem = nullptr;
     double *vel = nullptr, *fout = nullptr;
     int *id = nullptr, *nid = nullptr, *key = nullptr, *arank = nullptr, *perm = nullptr;
+    // pos0 / frc / mem stay in the slot order of the step that wrote them;
+    // prev[s] is the slot in that order of the particle now in slot s
+    // (lagged), until settle() gathers them into the current order
+    int *prev = nullptr;
+    bool lagged = false;
+    // graph replay of graph_steps steps (0: eager launches); ctl is the
+    // device-side step control the captured kernels read
+    int graph_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey{};
+    psim::StepCtl *ctl = nullptr;
     int *count = nullptr, *cstart = nullptr, *bsum = nullptr;
     psim::DevErr *err = nullptr;
     double *noise = nullptr;
@@ -90,8 +111,12 @@ inline unsigned blocks_for(long long n, int threads)
 {
     return (unsigned)((n + threads - 1) / threads);
 }
-// counts (already accumulated) -> cstart, scatter, canonical move
+// counts (already accumulated) -> cstart, scatter, canonical move of the
+// positions and ids (the state arrays become lagged)
 int launch_build(psim_ctx *c);
+// gather lagged pos0 / frc / mem into the current slot order (no-op when
+// they are in it already)
+int settle(psim_ctx *c);
 // nsteps of system::run with the context's step bookkeeping
 int run_steps(psim_ctx *c, int nsteps, const double *d_noise);
 int read_error(psim_ctx *c, int *code, int *i, int *j);

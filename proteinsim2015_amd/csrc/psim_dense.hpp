@@ -55,11 +55,14 @@ struct DenseSharedT {
     double shx[27], shy[27], shz[27];
 };
 using DenseShared = DenseSharedT<kDenseThreads>;
-// waves per SIMD for the kernels of force mask `mask`
-constexpr int dense_waves(int mask)
+// waves per SIMD for the kernels of force mask `mask` and compile-time LJ
+// exponent `ljn` (0: ljNum read at run time): LJ with AO, or Calibration with
+// LJ and a run-time exponent, need more than 72 VGPRs (6 waves)
+constexpr int dense_waves(int mask, int ljn = 0)
 {
-    return ((mask & F_LJ) && (mask & F_AO)) ? (PSIM_DENSE_WAVES < 6 ? PSIM_DENSE_WAVES : 6)
-                                            : PSIM_DENSE_WAVES;
+    const bool wide = ((mask & F_LJ) && (mask & F_AO)) ||
+                      ((mask & F_CAL) && (mask & F_LJ) && ljn == 0);
+    return wide ? (PSIM_DENSE_WAVES < 6 ? PSIM_DENSE_WAVES : 6) : PSIM_DENSE_WAVES;
 }
 // the workgroups that PSIM_DENSE_WAVES waves per SIMD make resident on one CU
 // (4 SIMDs) must also fit its 160 KiB of LDS
@@ -108,6 +111,16 @@ __device__ __forceinline__ void dense_pair(const Geo &g, const Pot &p,
     fz = fma(fr, dz, fz);
 }
 
+// a * b + c for a, b < 2^24 with b uniform over the wave (v_mad_u32_u24:
+// one full-rate instruction; the compiler otherwise folds the index
+// arithmetic into a 64-bit mad)
+__device__ __forceinline__ int mad_u24(int a, int b_uniform, int c)
+{
+    int r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b_uniform), "v"(c));
+    return r;
+}
+
 // Survivors of one batch of n <= 32 candidates j0, j0 + G, ...: bit
 // (n - 1 - k) of m is set when candidate j0 + k G passed the cull, so taking
 // the highest bit first keeps the candidate order (the bit with c leading
@@ -123,7 +136,7 @@ __device__ void dense_drain_mask(const Geo &g, const Pot &p, const DenseSharedT<
     while (m) {
         const int c = __clz(m);
         m ^= 0x80000000u >> c;
-        const int j = jc + __mul24(c, G);
+        const int j = mad_u24(c, G, jc);
         if (j == self) continue;
         dense_pair<MASK, T, LJN>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, j, fx, fy, fz, err,
                                  id_i, id, st);

@@ -48,6 +48,7 @@ struct Pot {
     double lj_four_n;       // 4.0*ljNum
     double lj_att;          // (-kT)*wellDepth*4*ljNum (the kernels' folded factors)
     double lj_rep;          // (-kT)*wellDepth*yukawaStrength
+    double lj_rep_lam;      // lj_rep * debyeLength
     double debye_len;
     double debye_inv;
     double yuk_a;
@@ -64,6 +65,21 @@ struct GB {
     double m;         // 1/mass
     double dt;
     double g2, g3, g4;  // velocityStep (:328-330)
+};
+
+// Step control kept in device memory so that a captured sequence of steps
+// can be replayed: the step kernels read it, k_ctl_advance at the end of each
+// step advances it.
+// (brownianIntegrator.cpp:183 first-step test, :288-292 velocity counter,
+// system.cpp:210 time advance)
+struct StepCtl {
+    unsigned long long step;   // Philox counter of the current step
+    double time;               // currentTime at the start of the step
+    double dt;
+    int vel_counter;
+    int vel_freq;
+    int first;                 // currentTime == 0
+    int do_vel;                // velocityStep this step
 };
 
 struct DevErr {
@@ -188,15 +204,17 @@ __device__ __forceinline__ double pow_int(double b, int e)
     return a;
 }
 
-// 1/sqrt(x) for normal x: hardware estimate (v_rsq_f64) refined by two
-// Newton steps y <- y + y (1/2 - (x/2) y^2); relative error ~1 ulp.
+// 1/sqrt(x) for normal x: the hardware estimate (v_rsq_f64, measured on
+// gfx950 within 2.5e8 ulp, i.e. ~6e-8 relative) refined by one third-order
+// step y <- y + y e (1/2 + 3/8 e), e = 1 - x y^2 (the series of
+// (1 - e)^(-1/2) to e^2), which leaves a relative error ~(5/16) e^3 below
+// the rounding of the last operations: within about 1 ulp, at five
+// instructions against the seven of two Newton steps.
 __device__ __forceinline__ double rsqrt_nr(double x)
 {
-    double y = __builtin_amdgcn_rsq(x);
-    const double h = 0.5 * x;
-    y = fma(y, fma(-h * y, y, 0.5), y);
-    y = fma(y, fma(-h * y, y, 0.5), y);
-    return y;
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-(x * y), y, 1.0);
+    return fma(y * e, fma(0.375, e, 0.5), y);
 }
 
 // e^{-a} for 0 <= a < 700: a = k ln2 + f with |f| <= ln2/2 (two-part ln2),
@@ -311,13 +329,14 @@ __device__ __forceinline__ double pair_fr(const Pot &p, double rinv, double r, d
     if (MASK & F_LJ) {
         // LJPotential.cpp:97-109: fNet = (-kT eps) (attract + repel) with
         // attract = (2 lj - 1) 4 n lj / r, repel = A e^{-r/lambda} (lambda + r)
-        // / r^2; the factor (-kT eps) folded into each term's constant
+        // / r^2; the factor (-kT eps) folded into each term's constant, and
+        // (lambda + r) / r = lambda / r + 1 taken as one fma
         const double lj = lj_power_x<LJN>((size * size) * r2inv, p.lj_num, size, rinv);
         double t = fma(2.0, lj, -1.0) * (p.lj_att * lj);
         if (p.yuk_a != 0.0) {
             const double ye = TAB ? exp_neg_tab(r * p.debye_inv, etab)
                                   : exp_neg(r * p.debye_inv);
-            t = fma(ye, rinv * (p.debye_len + r) * p.lj_rep, t);
+            t = fma(ye, fma(p.lj_rep_lam, rinv, p.lj_rep), t);
         }
         t *= r2inv;
         f = (MASK & F_CAL) ? f + t : t;

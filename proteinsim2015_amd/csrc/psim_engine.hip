@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 #include <cmath>
 #include <cstdio>
@@ -96,10 +97,33 @@ int launch_build(psim_ctx *c)
     }
     k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, c->key, c->arank, c->cstart, c->perm,
                                                   direct ? c->count : nullptr, nc);
-    k_move<<<blocks_for(n, 256), 256, 0, st>>>(n, c->perm, c->key, c->cstart, c->npos,
-                                               c->npos0, c->nfrc, c->nmem, c->nid, c->pos,
-                                               c->pos0, c->frc, c->mem, c->id);
+    // positions and ids move into the new order; the previous positions,
+    // forces and memory stay where this step wrote them (npos0 / nfrc /
+    // nmem, old order) and the next step reads them through prev
+    k_move_lagged<<<blocks_for(n, 256), 256, 0, st>>>(n, c->perm, c->key, c->cstart, c->npos,
+                                                      c->id, c->pos, c->nid, c->prev);
     CK(hipGetLastError());
+    std::swap(c->id, c->nid);
+    std::swap(c->pos0, c->npos0);
+    std::swap(c->frc, c->nfrc);
+    std::swap(c->mem, c->nmem);
+    c->lagged = true;
+    return 0;
+}
+
+int settle(psim_ctx *c)
+{
+    if (!c->lagged) return 0;
+    k_settle<<<blocks_for(c->n, 256), 256, 0, c->stream>>>(c->n, c->prev, c->pos0, c->frc,
+                                                           c->mem, c->npos0, c->nfrc, c->nmem);
+    CK(hipGetLastError());
+    // copied back rather than swapped: the buffers stay where a captured
+    // step sequence expects them
+    const size_t b = sizeof(double) * 3 * (size_t)c->n;
+    CK(hipMemcpyAsync(c->pos0, c->npos0, b, hipMemcpyDeviceToDevice, c->stream));
+    CK(hipMemcpyAsync(c->frc, c->nfrc, b, hipMemcpyDeviceToDevice, c->stream));
+    CK(hipMemcpyAsync(c->mem, c->nmem, b, hipMemcpyDeviceToDevice, c->stream));
+    c->lagged = false;
     return 0;
 }
 
@@ -219,53 +243,167 @@ void launch_table_mask(psim_ctx *c)
         c->geo, c->pot, c->tsorted, c->thi, c->tcse, c->fout, c->err);
 }
 
+// The step arguments of the context's current buffers.  ctl: the kernels
+// read the step index and the first-step / velocity-step flags from device
+// memory (graph replay) instead of from the arguments.
+StepArgs step_args(psim_ctx *c, const double *noise, const StepCtl *ctl)
+{
+    StepArgs a{};
+    a.pos = c->pos; a.pos0 = c->pos0; a.frc = c->frc; a.mem = c->mem;
+    a.id = c->id; a.cstart = c->cstart;
+    a.prev = c->lagged ? c->prev : nullptr;
+    a.npos = c->npos; a.npos0 = c->npos0; a.nfrc = c->nfrc; a.nmem = c->nmem;
+    a.nid = nullptr; a.vel = c->vel; a.key = c->key; a.arank = c->arank;
+    a.count = c->count; a.err = c->err;
+    a.pbc = c->pbc;
+    a.noise = noise;
+    a.seed = c->prm.seed;
+    a.step = c->step_index;
+    a.ctl = ctl;
+    return a;
+}
+
+// The first-step and velocity-step flags of the next step, advancing the
+// velocity counter (brownianIntegrator.cpp:183, 288, 292).
+void next_flags(psim_ctx *c, int *first, int *do_vel)
+{
+    *first = (c->time == 0) ? 1 : 0;
+    *do_vel = 0;
+    if (!*first) {
+        *do_vel = (c->prm.vel_freq == 0 || c->vel_counter == c->prm.vel_freq) ? 1 : 0;
+        if (c->vel_counter == c->prm.vel_freq) c->vel_counter = 0; else c->vel_counter++;
+    }
+}
+
+// One step's kernels: forces (+ G&B), separate G&B on the dense path, cell
+// rebuild.
+int enqueue_step(psim_ctx *c, const StepArgs &a, psim_ctx::Mark *mk)
+{
+    if (mk) {
+        hipEvent_t e = take_event(c, &mk->e0);
+        if (e) CK(hipEventRecord(e, c->stream));
+    }
+    int rc = launch_step(c, a);
+    if (rc) return rc;
+    if (mk) {
+        hipEvent_t e = take_event(c, &mk->e1);
+        if (e) CK(hipEventRecord(e, c->stream));
+    }
+    if (c->dense) {
+        k_integrate<<<blocks_for(c->n, 256), 256, 0, c->stream>>>(c->geo, c->gb, a);
+        CK(hipGetLastError());
+    }
+    if (mk) {
+        hipEvent_t e = take_event(c, &mk->e1b);
+        if (e) CK(hipEventRecord(e, c->stream));
+    }
+    rc = launch_build(c);
+    if (rc) return rc;
+    if (mk) {
+        hipEvent_t e = take_event(c, &mk->e2);
+        if (e) CK(hipEventRecord(e, c->stream));
+    }
+    return 0;
+}
+
+// What a captured step sequence bakes into its kernel arguments.
+GraphKey graph_key(const psim_ctx *c)
+{
+    GraphKey k{};
+    k.pos = c->pos; k.pos0 = c->pos0; k.frc = c->frc; k.mem = c->mem;
+    k.npos = c->npos; k.npos0 = c->npos0; k.nfrc = c->nfrc; k.nmem = c->nmem;
+    k.id = c->id; k.nid = c->nid; k.pbc = c->pbc;
+    k.geo = c->geo; k.pot = c->pot; k.gb = c->gb; k.seed = c->prm.seed;
+    return k;
+}
+
+__global__ void k_iota(int n, int *p)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) p[s] = s;
+}
+
+void drop_graph(psim_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+// graph_steps steps from one graph launch (captured on first use, and again
+// whenever the buffers or constants it baked in have changed).  The step
+// sequence swaps the lagged buffers once per step, so an even number of
+// steps leaves them where the capture found them.
+int replay_graph(psim_ctx *c)
+{
+    if (!c->lagged) {
+        // the captured steps read pos0 / frc / mem through prev
+        k_iota<<<blocks_for(c->n, 256), 256, 0, c->stream>>>(c->n, c->prev);
+        CK(hipGetLastError());
+        c->lagged = true;
+    }
+    const GraphKey key = graph_key(c);
+    if (c->gexec && std::memcmp(&key, &c->gkey, sizeof key) != 0) drop_graph(c);
+    // step control of the first captured step; k_ctl_advance prepares the rest
+    StepCtl h{};
+    h.step = c->step_index;
+    h.time = c->time;
+    h.dt = c->prm.dt;
+    h.vel_freq = c->prm.vel_freq;
+    next_flags(c, &h.first, &h.do_vel);
+    h.vel_counter = c->vel_counter;
+    CK(hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    if (!c->gexec) {
+        CK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = 0;
+        for (int k = 0; k < c->graph_steps && !rc; ++k) {
+            const StepArgs a = step_args(c, nullptr, c->ctl);
+            rc = enqueue_step(c, a, nullptr);
+            if (!rc) k_ctl_advance<<<1, 64, 0, c->stream>>>(c->ctl);
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        if (rc || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            std::fprintf(stderr, "psim_gpu: step capture failed: %s\n", hipGetErrorString(ce));
+            return rc ? rc : PSIM_E_HIP;
+        }
+        c->graph = g;
+        CK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->gkey = key;
+    }
+    CK(hipGraphLaunch(c->gexec, c->stream));
+    // the host's copy of the step control, as the eager loop leaves it
+    for (int k = 0; k < c->graph_steps; ++k) {
+        if (k > 0) {
+            int f, v;
+            next_flags(c, &f, &v);
+        }
+        c->step_index += 1;
+        c->time += c->prm.dt;   // system.cpp:210
+    }
+    return 0;
+}
+
 int run_steps(psim_ctx *c, int nsteps, const double *d_noise)
 {
     if (!c->have_state) return PSIM_E_ARG;
     const int n = c->n;
-    for (int s = 0; s < nsteps; ++s) {
-        StepArgs a{};
-        a.pos = c->pos; a.pos0 = c->pos0; a.frc = c->frc; a.mem = c->mem;
-        a.id = c->id; a.cstart = c->cstart;
-        a.npos = c->npos; a.npos0 = c->npos0; a.nfrc = c->nfrc; a.nmem = c->nmem;
-        a.nid = c->nid; a.vel = c->vel; a.key = c->key; a.arank = c->arank;
-        a.count = c->count; a.err = c->err;
-        a.pbc = c->pbc;
-        a.noise = d_noise ? d_noise + (size_t)s * 6 * n : nullptr;
-        a.seed = c->prm.seed;
-        a.step = c->step_index;
-        a.ctl = nullptr;
-        a.first = (c->time == 0) ? 1 : 0;   // brownianIntegrator.cpp:183
-        a.do_vel = 0;
-        if (!a.first) {
-            a.do_vel = (c->prm.vel_freq == 0 || c->vel_counter == c->prm.vel_freq) ? 1 : 0;
-            // :292
-            if (c->vel_counter == c->prm.vel_freq) c->vel_counter = 0; else c->vel_counter++;
+    int s = 0;
+    if (c->graph_steps > 0 && !d_noise && !c->profiling) {
+        for (; s + c->graph_steps <= nsteps; s += c->graph_steps) {
+            const int rc = replay_graph(c);
+            if (rc) return rc;
         }
+    }
+    for (; s < nsteps; ++s) {
+        StepArgs a = step_args(c, d_noise ? d_noise + (size_t)s * 6 * n : nullptr, nullptr);
+        next_flags(c, &a.first, &a.do_vel);
         psim_ctx::Mark mk{};
-        if (c->profiling) {
-            hipEvent_t e = take_event(c, &mk.e0);
-            if (e) CK(hipEventRecord(e, c->stream));
-        }
-        int rc = launch_step(c, a);
+        int rc = enqueue_step(c, a, c->profiling ? &mk : nullptr);
         if (rc) return rc;
         if (c->profiling) {
-            hipEvent_t e = take_event(c, &mk.e1);
-            if (e) CK(hipEventRecord(e, c->stream));
-        }
-        if (c->dense) {
-            k_integrate<<<blocks_for(n, 256), 256, 0, c->stream>>>(c->geo, c->gb, a);
-            CK(hipGetLastError());
-        }
-        if (c->profiling) {
-            hipEvent_t e = take_event(c, &mk.e1b);
-            if (e) CK(hipEventRecord(e, c->stream));
-        }
-        rc = launch_build(c);
-        if (rc) return rc;
-        if (c->profiling) {
-            hipEvent_t e = take_event(c, &mk.e2);
-            if (e) CK(hipEventRecord(e, c->stream));
             c->marks.push_back(mk);
             if (c->ev_used > 3000) {
                 rc = collect_times(c);
@@ -359,7 +497,7 @@ int set_state(psim_ctx *c, const double *pos, const double *pos0, const double *
     if ((rc = soa(vel, c->vel, false, false))) return rc;   // velocities by original index
     if ((rc = soa(frc, c->nfrc, false, true))) return rc;
     if ((rc = soa(memx, c->nmem, false, true))) return rc;
-    CK(hipMemcpyAsync(c->nid, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    CK(hipMemcpyAsync(c->id, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
     CK(hipMemsetAsync(c->count, 0, sizeof(int) * c->ncell, c->stream));
     k_hash_count<<<blocks_for(n, 256), 256, 0, c->stream>>>(c->geo, c->npos, c->key,
                                                             c->arank, c->count);
@@ -375,6 +513,7 @@ int set_state(psim_ctx *c, const double *pos, const double *pos0, const double *
 }
 
 int launch_build(psim_ctx *c) { return ::launch_build(c); }
+int settle(psim_ctx *c) { return ::settle(c); }
 int run_steps(psim_ctx *c, int nsteps, const double *d_noise)
 {
     return ::run_steps(c, nsteps, d_noise);
@@ -445,6 +584,8 @@ int psim_gpu_create(const psim_params *params, psim_ctx **out)
     if (!rc) rc = dalloc(&c->key, c->n);
     if (!rc) rc = dalloc(&c->arank, c->n);
     if (!rc) rc = dalloc(&c->perm, c->n);
+    if (!rc) rc = dalloc(&c->prev, c->n);
+    if (!rc) rc = dalloc(&c->ctl, 1);
     if (!rc) rc = dalloc(&c->count, c->ncell);
     if (!rc) rc = dalloc(&c->cstart, (size_t)c->ncell + 1);
     if (!rc) rc = dalloc(&c->bsum, (size_t)c->nscan);
@@ -459,6 +600,15 @@ int psim_gpu_create(const psim_params *params, psim_ctx **out)
         return rc;
     }
     *out = c;
+    return 0;
+}
+
+int psim_gpu_set_graph(psim_ctx *c, int steps_per_graph)
+{
+    if (!c || steps_per_graph < 0 || (steps_per_graph & 1)) return PSIM_E_ARG;
+    CK(hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    c->graph_steps = steps_per_graph;
     return 0;
 }
 
@@ -486,9 +636,10 @@ void psim_gpu_destroy(psim_ctx *c)
 {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
     void *bufs[] = {c->pos, c->pos0, c->frc, c->mem, c->npos, c->npos0, c->nfrc,
                     c->nmem, c->vel, c->fout, c->id, c->nid, c->key, c->arank,
-                    c->perm, c->count, c->cstart, c->bsum, c->err, c->noise,
+                    c->perm, c->prev, c->ctl, c->count, c->cstart, c->bsum, c->err, c->noise,
                     c->tsorted, c->thi, c->tcse, c->st_count, c->st_hash, c->pbc,
                     c->track0, c->hg.key,
                     c->hg.rank, c->hg.perm, c->hg.count, c->hg.cstart, c->hg.bsum,
@@ -517,6 +668,8 @@ int psim_gpu_get_state(psim_ctx *c, double *pos, double *pos0, double *vel, doub
                        int *vel_counter, unsigned long long *step_index)
 {
     if (!c || !c->have_state) return PSIM_E_ARG;
+    int rs = psim::eng::settle(c);
+    if (rs) return rs;
     const int n = c->n;
     std::vector<int> ids(n);
     std::vector<double> h(3 * (size_t)n);

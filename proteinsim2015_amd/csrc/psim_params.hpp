@@ -107,6 +107,7 @@ inline void make_constants(const psim_params &P, Geo &g, Pot &p, GB &b)
     p.yuk_a = P.yukawa_strength;
     p.lj_att = p.lj_neg_kt_eps * p.lj_four_n;
     p.lj_rep = p.lj_neg_kt_eps * p.yuk_a;
+    p.lj_rep_lam = p.lj_rep * P.debye_length;
     // AOPotential.cpp:50-55
     double q = (rc / (rc - 1.0));
     double a1 = -P.kT * P.ao_well_depth * q * q * q;

@@ -67,20 +67,6 @@ inline PathChoice choose_path(const psim_params &P, const Geo &g)
     return c;
 }
 
-// Step control kept in device memory so that a captured sequence of steps
-// can be replayed: the step kernels read it, the reorder kernel advances it.
-// (brownianIntegrator.cpp:183 first-step test, :288-292 velocity counter,
-// system.cpp:210 time advance)
-struct StepCtl {
-    unsigned long long step;   // Philox counter of the current step
-    double time;               // currentTime at the start of the step
-    double dt;
-    int vel_counter;
-    int vel_freq;
-    int first;                 // currentTime == 0
-    int do_vel;                // velocityStep this step
-};
-
 __device__ __forceinline__ void ctl_prepare(StepCtl *c)
 {
     c->first = (c->time == 0) ? 1 : 0;
@@ -91,11 +77,23 @@ __device__ __forceinline__ void ctl_prepare(StepCtl *c)
     }
 }
 
+// End of step: advance the clock and set the flags of the next step
+// (system.cpp:210; brownianIntegrator.cpp:183, 288-292).
+__global__ void k_ctl_advance(StepCtl *c)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->step += 1;
+        c->time += c->dt;
+        ctl_prepare(c);
+    }
+}
+
 struct StepArgs {
     const double *pos, *pos0, *frc, *mem;
     const int *id, *cstart;
+    const int *prev;     // slot of each particle in pos0 / frc / mem (null: the same slot)
     double *npos, *npos0, *nfrc, *nmem;
-    int *nid;
+    int *nid;            // copy of id in the current order (null: not kept)
     double *vel;         // [3][N] by original index
     int *key, *arank, *count;
     DevErr *err;
@@ -126,9 +124,11 @@ __device__ __forceinline__ void integrate_one(const Geo &g, const GB &gb, const 
     }
     const double xc[3] = {a.pos[s], a.pos[N + s], a.pos[2 * N + s]};
     const double F[3] = {fx, fy, fz};
+    // the previous position, force and memory where the last step left them
+    const int ps = a.prev ? a.prev[s] : s;
     double x0c[3], xn[3], M[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) x0c[d] = a.pos0[d * N + s];
+    for (int d = 0; d < 3; ++d) x0c[d] = a.pos0[d * N + ps];
     if (first) {
         // firstStep :200-218
 #pragma unroll
@@ -143,8 +143,8 @@ __device__ __forceinline__ void integrate_one(const Geo &g, const GB &gb, const 
         double Mo[3], Fo[3], Cr[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            Mo[d] = a.mem[d * N + s];
-            Fo[d] = a.frc[d * N + s];
+            Mo[d] = a.mem[d * N + ps];
+            Fo[d] = a.frc[d * N + ps];
             Cr[d] = gb.sig2 * ((gb.corr * Mo[d]) + (gb.dev * z[d]));
             M[d] = z[3 + d];
         }
@@ -195,7 +195,7 @@ __device__ __forceinline__ void integrate_one(const Geo &g, const GB &gb, const 
         if (WRITE_F) a.nfrc[d * N + s] = F[d];
         a.nmem[d * N + s] = M[d];
     }
-    a.nid[s] = o;
+    if (a.nid) a.nid[s] = o;
     if (bad) report_error(a.err, 7703, o, -1);
     int k = g.c0 + cell_1d(xw[0], g.cell, g.S) + g.S * cell_1d(xw[1], g.cell, g.S)
           + g.S2 * cell_1d(xw[2], g.cell, g.S);
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_step_brick(Geo g, Pot p, GB g
 // latency-bound; the separate kernel runs it at full occupancy.)
 template <int MASK, int LJN>
 __global__ __launch_bounds__(kDenseThreads)
-__attribute__((amdgpu_waves_per_eu(dense_waves(MASK)))) void k_step_dense(
+__attribute__((amdgpu_waves_per_eu(dense_waves(MASK, LJN)))) void k_step_dense(
     Geo g, Pot p, double radius, const double *pos, const int *id, const int *cstart,
     double *nfrc, DevErr *err, int per_xcd)
 {
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_forces_brick(Geo g, Pot p, do
 
 template <int MASK, int LJN, bool AOS = false>
 __global__ __launch_bounds__(kDenseThreads)
-__attribute__((amdgpu_waves_per_eu(dense_waves(MASK)))) void k_forces_dense(
+__attribute__((amdgpu_waves_per_eu(dense_waves(MASK, LJN)))) void k_forces_dense(
     Geo g, Pot p, double radius, const double *pos, const int *id, const int *cstart,
     double *fout, DevErr *err, int per_xcd)
 {

@@ -5,7 +5,8 @@ and their AMDGPU metadata printed with llvm-readelf.  The test holds the
 properties the launch code and the measured throughput rely on: no kernel
 spills to scratch memory, and the cell-per-workgroup pair kernels stay within
 the VGPR and LDS budgets of the waves per SIMD they are compiled for (7, or 6
-with LJ and AO both enabled: 512 / waves VGPRs rounded down to a multiple of
+with LJ and AO both enabled or Calibration with LJ of a run-time exponent:
+512 / waves VGPRs rounded down to a multiple of
 8, and that many 4-wave workgroups in a CU's 160 KiB of LDS)."""
 import os
 import re
@@ -98,8 +99,9 @@ def test_dense_kernels_fit_their_waves(kernels, kernel):
         if not m:
             continue
         seen += 1
-        mask = int(m.group(1))
-        waves = 6 if (mask & 2 and mask & 4) else 7
+        mask, ljn = int(m.group(1)), int(m.group(2))
+        wide = (mask & 2 and mask & 4) or (mask & 1 and mask & 2 and ljn == 0)
+        waves = 6 if wide else 7
         assert v["vgpr_count"] <= (512 // waves) // 8 * 8, (name, v)
         assert waves * v["group_segment_fixed_size"] <= 160 * 1024, (name, v)
     assert seen >= 16
