@@ -149,6 +149,15 @@ int psim_gpu_step(psim_ctx *ctx, int nsteps, const double *noise);
 /* As psim_gpu_step with noise already resident in device memory. */
 int psim_gpu_step_device_noise(psim_ctx *ctx, int nsteps, const double *d_noise);
 
+/* steps_per_graph > 0 (even): psim_gpu_step_device_noise and psim_gpu_step
+ * without injected noise run whole blocks of that many steps as one replay
+ * of a captured hipGraph (captured on first use, again after set_state,
+ * load, set_force_params or track_start), the rest eagerly; 0: every step
+ * eagerly.  Results are the same bits either way; kernel timing
+ * (psim_gpu_set_profiling) runs eagerly.  No reference counterpart (the
+ * reference runs one CPU loop per step, system.cpp:187-211). */
+int psim_gpu_set_graph(psim_ctx *ctx, int steps_per_graph);
+
 /* The parameters the context was created with (after psim_gpu_set_force_params). */
 int psim_gpu_get_params(psim_ctx *ctx, psim_params *out);
 

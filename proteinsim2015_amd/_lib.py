@@ -75,7 +75,7 @@ EXPORTS = (
     "psim_write_recovery", "psim_read_recovery", "psim_write_xyz", "psim_gpu_save",
     "psim_gpu_load", "psim_write_sys_config", "psim_gpu_kernel_info",
     "psim_gpu_pair_stats", "psim_ens_pair_stats", "psim_gpu_forces_noalloc",
-    "psim_int_create", "psim_int_destroy", "psim_int_next",
+    "psim_int_create", "psim_int_destroy", "psim_int_next", "psim_gpu_set_graph",
 )
 
 _vp = C.c_void_p
@@ -138,6 +138,7 @@ _SIGS = {
     "psim_ens_pair_stats": ([_vp, _ip, _up], C.c_int),
     "psim_gpu_forces_noalloc": ([C.POINTER(PsimParams), _vp, _vp, _vp], C.c_int),
     "psim_int_create": ([C.POINTER(PsimParams), C.c_int, C.POINTER(_vp)], C.c_int),
+    "psim_gpu_set_graph": ([_vp, C.c_int], C.c_int),
     "psim_int_destroy": ([_vp], None),
     "psim_int_next": ([_vp, C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip], C.c_int),
     "psim_gpu_get_tracker": ([_vp, _dp, _ip], C.c_int),

@@ -67,7 +67,9 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
     if code:
         raise RuntimeError(f"ensemble physics error {code}")
     batched = ntot * steps / (t2 - t1)
-    # ---- one system per launch: each replica its own context and stream
+    # ---- one system per launch: each replica its own context and stream,
+    # launched eagerly (4 launches per step) and as one hipGraph replay of
+    # baseline_steps steps per replica (the single-system engine's best mode)
     systems = []
     for r, p, v in zip(reps, pos, vel):
         s = E.GpuSystem(r.geometry, r.forces, integ, kernel_path="auto")
@@ -83,12 +85,24 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
     for s in systems:
         s.sync()
     t4 = time.perf_counter()
+    for s in systems:
+        s.set_graph(baseline_steps)
+        s.step_device_noise(baseline_steps, 0)    # captures, then replays once
+    for s in systems:
+        s.sync()
+    t5 = time.perf_counter()
+    for s in systems:
+        s.step_device_noise(baseline_steps, 0)
+    for s in systems:
+        s.sync()
+    t6 = time.perf_counter()
     codes = [s.last_error()[0] for s in systems]
     for s in systems:
         s.close()
     if any(codes):
         raise RuntimeError(f"per-replica physics error {max(codes)}")
     single = ntot * baseline_steps / (t4 - t3)
+    single_graph = ntot * baseline_steps / (t6 - t5)
     return {
         "replicas": nrep, "particles": ntot, "steps": steps,
         "batched_particle_steps_per_s": batched,
@@ -96,7 +110,10 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
         "per_replica_particle_steps_per_s": single,
         "per_replica_ms_per_step": (t4 - t3) / baseline_steps * 1e3,
         "per_replica_steps_timed": baseline_steps,
+        "per_replica_graph_particle_steps_per_s": single_graph,
+        "per_replica_graph_ms_per_step": (t6 - t5) / baseline_steps * 1e3,
         "speedup": batched / single,
+        "speedup_vs_graph_baseline": batched / single_graph,
         "autotuned_kernel": kernel,
         "autotuned_tile": tile,
         "graph_steps": graph_steps,

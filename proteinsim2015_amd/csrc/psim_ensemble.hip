@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "psim_build.hpp"
@@ -55,17 +56,28 @@ __device__ __forceinline__ int replica_of(const int *__restrict__ p0, int nrep, 
     return lo;
 }
 
+// Thread-per-particle step: workgroup `it` of the launch covers slots
+// p0[r] + (it - blk0[r]) BLOCK .. of replica r (blk0 = prefix of the
+// replicas' workgroup counts), so a workgroup never spans two replicas and
+// the replica's geometry and constants are the same for every lane (scalar
+// loads, no per-lane replica search).
 template <int MASK, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_ens_step(const RepGeo *__restrict__ reps,
-                                                    const int *__restrict__ p0, int nrep,
-                                                    int ntot, GB gb, StepArgs a)
+                                                    const int *__restrict__ p0,
+                                                    const int *__restrict__ blk0, int nrep,
+                                                    int nblk, int per_xcd, int ntot, GB gb,
+                                                    StepArgs a)
 {
     __shared__ unsigned hits[kHitCap * BLOCK];
-    const int s = xcd_block() * BLOCK + threadIdx.x;
-    if (s >= ntot) return;
-    const int r = replica_of(p0, nrep, a.id[s]);
-    const Geo g = reps[r].g;
-    const Pot p = reps[r].p;
+    const int it = xcd_cell(blockIdx.x, per_xcd);
+    if (it >= nblk) return;
+    const int r = replica_of(blk0, nrep, it);
+    const int s = p0[r] + (it - blk0[r]) * BLOCK + (int)threadIdx.x;
+    if (s >= p0[r + 1]) return;
+    // references into the replica table: local copies passed on by
+    // reference would live in scratch memory
+    const Geo &g = reps[r].g;
+    const Pot &p = reps[r].p;
     const double radius = reps[r].radius;
     const int N = ntot;
     const double xi = a.pos[s], yi = a.pos[N + s], zi = a.pos[2 * N + s];
@@ -114,16 +126,20 @@ __global__ __launch_bounds__(kBrickThreads) void k_ens_step_brick(
 // shapes and acceptance code; see NbrStats).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_ens_stats(const RepGeo *__restrict__ reps,
-                                                     const int *__restrict__ p0, int nrep,
-                                                     int ntot, const double *pos, const int *id,
+                                                     const int *__restrict__ p0,
+                                                     const int *__restrict__ blk0, int nrep,
+                                                     int nblk, int per_xcd, int ntot,
+                                                     const double *pos, const int *id,
                                                      const int *cstart, NbrStatsOut so)
 {
     __shared__ unsigned hits[kHitCap * BLOCK];
-    const int s = xcd_block() * BLOCK + threadIdx.x;
-    if (s >= ntot) return;
-    const int r = replica_of(p0, nrep, id[s]);
-    const Geo g = reps[r].g;
-    const Pot p = reps[r].p;
+    const int it = xcd_cell(blockIdx.x, per_xcd);
+    if (it >= nblk) return;
+    const int r = replica_of(blk0, nrep, it);
+    const int s = p0[r] + (it - blk0[r]) * BLOCK + (int)threadIdx.x;
+    if (s >= p0[r + 1]) return;
+    const Geo &g = reps[r].g;
+    const Pot &p = reps[r].p;
     const double radius = reps[r].radius;
     const int N = ntot;
     const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
@@ -164,6 +180,12 @@ __global__ __launch_bounds__(kBrickThreads) void k_ens_stats_brick(
                                                 sh, epi, &so);
 }
 
+__global__ void k_iota_ens(int n, int *p)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) p[s] = s;
+}
+
 // Initial hash of every particle into its replica's cells.
 __global__ void k_ens_hash(const RepGeo *__restrict__ reps, const int *__restrict__ p0,
                            int nrep, int ntot, const double *pos, const int *id, int *key,
@@ -176,17 +198,6 @@ __global__ void k_ens_hash(const RepGeo *__restrict__ reps, const int *__restric
                 + g.S2 * cell_1d(pos[2 * ntot + s], g.cell, g.S);
     key[s] = k;
     arank[s] = wave_aggregated_rank(count, k);
-}
-
-// End of step: advance the clock and set the flags of the next step
-// (system.cpp:210; brownianIntegrator.cpp:183, 288-292).
-__global__ void k_ctl_advance(StepCtl *c)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        c->step += 1;
-        c->time += c->dt;
-        ctl_prepare(c);
-    }
 }
 
 constexpr int kEnsBlocks[3] = {64, 128, 256};
@@ -205,6 +216,8 @@ struct psim_ens {
     int target = 192;    // brick kernel: home particles per brick
     int nitems = 0;      // bricks over all replicas
     int *item0 = nullptr;
+    int nblk = 0;        // thread-per-particle workgroups over all replicas
+    int *blk0 = nullptr; // prefix of the replicas' workgroup counts
     std::vector<RepGeo> h_reps;
     std::vector<int> h_occ_n;
     hipStream_t stream = nullptr;
@@ -217,14 +230,21 @@ struct psim_ens {
     double *vel = nullptr;
     int *id = nullptr, *nid = nullptr, *key = nullptr, *arank = nullptr, *perm = nullptr;
     int *count = nullptr, *cstart = nullptr, *bsum = nullptr;
+    // pos0 / frc / mem stay in the slot order of the step that wrote them;
+    // prev[s] is the slot in that order of the particle now in slot s
+    int *prev = nullptr;
+    bool lagged = false;
     DevErr *err = nullptr;
     StepCtl *ctl = nullptr;
     int *st_count = nullptr;                 // neighbour-set statistics
     unsigned long long *st_hash = nullptr;
     bool have_state = false;
     int graph_steps = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // captured step sequences, one per arrangement of the swapped buffers
+    // they start from (an odd number of steps alternates between two)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    const void *gkey[2] = {nullptr, nullptr};   // pos0 buffer at the capture's start
 };
 
 #define ECK(x)                                                                   \
@@ -257,8 +277,9 @@ StepArgs ens_args(psim_ens *e)
     StepArgs a{};
     a.pos = e->pos; a.pos0 = e->pos0; a.frc = e->frc; a.mem = e->mem;
     a.id = e->id; a.cstart = e->cstart;
+    a.prev = e->lagged ? e->prev : nullptr;
     a.npos = e->npos; a.npos0 = e->npos0; a.nfrc = e->nfrc; a.nmem = e->nmem;
-    a.nid = e->nid; a.vel = e->vel; a.key = e->key; a.arank = e->arank;
+    a.nid = nullptr; a.vel = e->vel; a.key = e->key; a.arank = e->arank;
     a.count = e->count; a.err = e->err;
     a.noise = nullptr;
     a.seed = e->common.seed;
@@ -269,8 +290,9 @@ StepArgs ens_args(psim_ens *e)
 template <int MASK, int BLOCK>
 void launch_ens_pair(psim_ens *e, const StepArgs &a)
 {
-    k_ens_step<MASK, BLOCK><<<xcd_grid(eblocks(e->ntot, BLOCK)), BLOCK, 0, e->stream>>>(
-        e->reps, e->p0, e->nrep, e->ntot, e->gb, a);
+    const int per = (e->nblk + 7) / 8;
+    k_ens_step<MASK, BLOCK><<<8 * per, BLOCK, 0, e->stream>>>(
+        e->reps, e->p0, e->blk0, e->nrep, e->nblk, per, e->ntot, e->gb, a);
 }
 
 template <int MASK>
@@ -302,9 +324,49 @@ void launch_build_ens(psim_ens *e)
     }
     k_scatter<<<eblocks(n, 256), 256, 0, st>>>(n, e->key, e->arank, e->cstart, e->perm,
                                                direct ? e->count : nullptr, nc);
-    k_move<<<eblocks(n, 256), 256, 0, st>>>(n, e->perm, e->key, e->cstart, e->npos, e->npos0,
-                                            e->nfrc, e->nmem, e->nid, e->pos, e->pos0, e->frc,
-                                            e->mem, e->id);
+    // positions and ids move into the new order; the previous positions,
+    // forces and memory stay where this step wrote them, read through prev
+    k_move_lagged<<<eblocks(n, 256), 256, 0, st>>>(n, e->perm, e->key, e->cstart, e->npos,
+                                                   e->id, e->pos, e->nid, e->prev);
+    std::swap(e->id, e->nid);
+    std::swap(e->pos0, e->npos0);
+    std::swap(e->frc, e->nfrc);
+    std::swap(e->mem, e->nmem);
+    e->lagged = true;
+}
+
+// The lagged arrays gathered into the current slot order, copied back so
+// that the buffers stay where a captured step sequence expects them.
+int settle_ens(psim_ens *e)
+{
+    if (!e->lagged) return 0;
+    const int n = e->ntot;
+    k_settle<<<eblocks(n, 256), 256, 0, e->stream>>>(n, e->prev, e->pos0, e->frc, e->mem,
+                                                     e->npos0, e->nfrc, e->nmem);
+    ECK(hipGetLastError());
+    const size_t b = sizeof(double) * 3 * (size_t)n;
+    ECK(hipMemcpyAsync(e->pos0, e->npos0, b, hipMemcpyDeviceToDevice, e->stream));
+    ECK(hipMemcpyAsync(e->frc, e->nfrc, b, hipMemcpyDeviceToDevice, e->stream));
+    ECK(hipMemcpyAsync(e->mem, e->nmem, b, hipMemcpyDeviceToDevice, e->stream));
+    e->lagged = false;
+    return 0;
+}
+
+// Workgroups of `block` lanes per replica for the thread-per-particle
+// kernel, and their prefix.
+int set_blocks(psim_ens *e, int block)
+{
+    std::vector<int> b0(e->nrep + 1);
+    int nb = 0;
+    for (int r = 0; r < e->nrep; ++r) {
+        b0[r] = nb;
+        nb += (e->h_occ_n[r] + block - 1) / block;
+    }
+    b0[e->nrep] = nb;
+    ECK(hipMemcpy(e->blk0, b0.data(), sizeof(int) * (e->nrep + 1), hipMemcpyHostToDevice));
+    e->block = block;
+    e->nblk = nb;
+    return 0;
 }
 
 // One full step, enqueued on the ensemble's stream (also used while
@@ -330,10 +392,13 @@ int enqueue_step(psim_ens *e)
 
 void drop_graph(psim_ens *e)
 {
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
-    e->exec = nullptr;
-    e->graph = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        if (e->exec[k]) (void)hipGraphExecDestroy(e->exec[k]);
+        if (e->graph[k]) (void)hipGraphDestroy(e->graph[k]);
+        e->exec[k] = nullptr;
+        e->graph[k] = nullptr;
+        e->gkey[k] = nullptr;
+    }
 }
 
 // Cut every replica into bricks of about `target` home particles and upload
@@ -358,9 +423,30 @@ int set_bricks(psim_ens *e, int target)
     return 0;
 }
 
-int capture_graph(psim_ens *e)
+// The captured sequence that starts from the current arrangement of the
+// swapped buffers (captured on first use).  Capturing enqueues nothing but
+// advances the host's buffer bookkeeping as the replay will.
+int graph_for_now(psim_ens *e, hipGraphExec_t *out)
 {
-    drop_graph(e);
+    if (!e->lagged) {
+        k_iota_ens<<<eblocks(e->ntot, 256), 256, 0, e->stream>>>(e->ntot, e->prev);
+        ECK(hipGetLastError());
+        e->lagged = true;
+    }
+    const void *key = e->pos0;
+    for (int k = 0; k < 2; ++k)
+        if (e->exec[k] && e->gkey[k] == key) {
+            *out = e->exec[k];
+            // the buffers the replay leaves behind
+            for (int q = 0; q < e->graph_steps; ++q) {
+                std::swap(e->id, e->nid);
+                std::swap(e->pos0, e->npos0);
+                std::swap(e->frc, e->nfrc);
+                std::swap(e->mem, e->nmem);
+            }
+            return 0;
+        }
+    const int slot = e->exec[0] ? 1 : 0;
     ECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
     for (int k = 0; k < e->graph_steps && !rc; ++k) rc = enqueue_step(e);
@@ -374,8 +460,10 @@ int capture_graph(psim_ens *e)
         std::fprintf(stderr, "psim_ens: graph capture failed: %s\n", hipGetErrorString(ce));
         return PSIM_E_HIP;
     }
-    e->graph = g;
-    ECK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    e->graph[slot] = g;
+    ECK(hipGraphInstantiate(&e->exec[slot], e->graph[slot], nullptr, nullptr, 0));
+    e->gkey[slot] = key;
+    *out = e->exec[slot];
     return 0;
 }
 
@@ -453,7 +541,7 @@ int psim_ens_create(const psim_params *common, const psim_replica *reps, int nre
                         &e->nmem, &e->vel};
     for (double **b : dbufs)
         if (!rc) rc = ealloc(b, n3);
-    int **ibufs[] = {&e->id, &e->nid, &e->key, &e->arank, &e->perm};
+    int **ibufs[] = {&e->id, &e->nid, &e->key, &e->arank, &e->perm, &e->prev};
     for (int **b : ibufs)
         if (!rc) rc = ealloc(b, (size_t)e->ntot);
     if (!rc) rc = ealloc(&e->count, (size_t)e->ncell);
@@ -464,10 +552,12 @@ int psim_ens_create(const psim_params *common, const psim_replica *reps, int nre
     if (!rc) rc = ealloc(&e->reps, (size_t)nrep);
     if (!rc) rc = ealloc(&e->p0, (size_t)nrep + 1);
     if (!rc) rc = ealloc(&e->item0, (size_t)nrep + 1);
+    if (!rc) rc = ealloc(&e->blk0, (size_t)nrep + 1);
     e->h_reps = h;
     e->h_occ_n.resize(nrep);
     for (int r = 0; r < nrep; ++r) e->h_occ_n[r] = reps[r].n_particles;
     if (!rc) rc = set_bricks(e, e->target);
+    if (!rc) rc = set_blocks(e, e->block);
     if (!rc && hipMemcpy(e->p0, e->h_p0.data(), sizeof(int) * (nrep + 1), hipMemcpyHostToDevice) != hipSuccess)
         rc = PSIM_E_HIP;
     if (!rc && hipMemset(e->count, 0, sizeof(int) * e->ncell) != hipSuccess) rc = PSIM_E_HIP;
@@ -488,7 +578,7 @@ void psim_ens_destroy(psim_ens *e)
     void *bufs[] = {e->pos, e->pos0, e->frc, e->mem, e->npos, e->npos0, e->nfrc, e->nmem,
                     e->vel, e->id, e->nid, e->key, e->arank, e->perm, e->count, e->cstart,
                     e->bsum, e->err, e->ctl, e->reps, e->p0, e->item0, e->st_count,
-                    e->st_hash};
+                    e->st_hash, e->blk0, e->prev};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -530,7 +620,7 @@ int psim_ens_set_state(psim_ens *e, const double *pos, const double *vel, double
     if ((rc = up(nullptr, e->nmem, false))) return rc;
     std::vector<int> ids(n);
     for (int i = 0; i < n; ++i) ids[i] = i;
-    ECK(hipMemcpy(e->nid, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    ECK(hipMemcpy(e->id, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     ECK(hipMemset(e->count, 0, sizeof(int) * e->ncell));
     StepCtl c{};
     c.step = 0;
@@ -547,7 +637,7 @@ int psim_ens_set_state(psim_ens *e, const double *pos, const double *vel, double
     }
     ECK(hipMemcpy(e->ctl, &c, sizeof(c), hipMemcpyHostToDevice));
     k_ens_hash<<<eblocks(n, 256), 256, 0, e->stream>>>(e->reps, e->p0, e->nrep, n, e->npos,
-                                                       e->nid, e->key, e->arank, e->count);
+                                                       e->id, e->key, e->arank, e->count);
     launch_build_ens(e);
     ECK(hipGetLastError());
     ECK(hipStreamSynchronize(e->stream));
@@ -559,6 +649,8 @@ int psim_ens_get_state(psim_ens *e, double *pos, double *pos0, double *vel, doub
 {
     if (!e || !e->have_state) return PSIM_E_ARG;
     const int n = e->ntot;
+    int rs = settle_ens(e);
+    if (rs) return rs;
     std::vector<int> ids(n);
     std::vector<double> h(3 * (size_t)n);
     ECK(hipStreamSynchronize(e->stream));
@@ -593,12 +685,12 @@ int psim_ens_step(psim_ens *e, int nsteps)
     if (!e || nsteps < 0 || !e->have_state) return PSIM_E_ARG;
     int done = 0;
     if (e->graph_steps > 0) {
-        if (!e->exec) {
-            int rc = capture_graph(e);
+        for (; done + e->graph_steps <= nsteps; done += e->graph_steps) {
+            hipGraphExec_t x = nullptr;
+            int rc = graph_for_now(e, &x);
             if (rc) return rc;
+            ECK(hipGraphLaunch(x, e->stream));
         }
-        for (; done + e->graph_steps <= nsteps; done += e->graph_steps)
-            ECK(hipGraphLaunch(e->exec, e->stream));
     }
     for (; done < nsteps; ++done) {
         int rc = enqueue_step(e);
@@ -642,9 +734,10 @@ int psim_ens_pair_stats(psim_ens *e, int *count_by_orig, unsigned long long *has
         k_ens_stats_brick<<<8 * per, kBrickThreads, 0, e->stream>>>(
             e->reps, e->item0, e->nrep, e->nitems, per, e->pos, e->id, e->cstart, so);
     } else {
+        const int per = (e->nblk + 7) / 8;
 #define PSIM_ENS_STATS(BL)                                                                 \
-    k_ens_stats<BL><<<xcd_grid(eblocks(n, BL)), BL, 0, e->stream>>>(e->reps, e->p0, e->nrep, n, \
-                                                                    e->pos, e->id, e->cstart, so)
+    k_ens_stats<BL><<<8 * per, BL, 0, e->stream>>>(e->reps, e->p0, e->blk0, e->nrep, e->nblk, \
+                                                   per, n, e->pos, e->id, e->cstart, so)
         switch (e->block) {
         case 64: PSIM_ENS_STATS(64); break;
         case 128: PSIM_ENS_STATS(128); break;
@@ -683,7 +776,7 @@ int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_kernel, int *cho
         if (brick) {
             if ((rc = set_bricks(e, b))) return rc;
         } else {
-            e->block = b;
+            if ((rc = set_blocks(e, b))) return rc;
         }
         rc = enqueue_step(e);
         if (rc) return rc;
@@ -705,7 +798,8 @@ int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_kernel, int *cho
         int rc = set_bricks(e, best_block);
         if (rc) return rc;
     } else {
-        e->block = best_block;
+        int rc = set_blocks(e, best_block);
+        if (rc) return rc;
     }
     drop_graph(e);   // recaptured with the chosen kernel on the next step
     (void)hipEventDestroy(t0);

@@ -380,6 +380,11 @@ class GpuSystem:
                                                   C.c_void_p(device_ptr)),
               "psim_gpu_step_device_noise")
 
+    def set_graph(self, steps_per_graph: int) -> None:
+        """Replay blocks of ``steps_per_graph`` (even) steps as one captured
+        hipGraph when stepping with in-kernel noise (0: eager launches)."""
+        check(self.lib.psim_gpu_set_graph(self._ctx, int(steps_per_graph)), "psim_gpu_set_graph")
+
     def kernel_info(self) -> dict:
         """The pair kernel this context runs ("sparse", "dense" or "brick"),
         the brick's home cells and the lanes per particle."""
