@@ -73,6 +73,9 @@ def parse():
     ap.add_argument("--graph", type=int, default=0,
                     help="replay the headline steps as hipGraphs of this many (even) steps; "
                          "the kernel split then comes from a separate profiled pass")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every step of the smaller workloads eagerly instead of as "
+                         "hipGraph replays (for tracing under a profiler)")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 512-replica batched-ensemble measurement")
     ap.add_argument("--no-adjoint", action="store_true",
@@ -245,7 +248,7 @@ def valu_from_profiles(prof, pair_ms):
                     "near 1 means VALU-issue bound"}
 
 
-def small_leg(name: str, steps: int, warmup: int = 10) -> dict:
+def small_leg(name: str, steps: int, warmup: int = 10, eager: bool = False) -> dict:
     """One of the smaller plain-engine workloads, timed like the headline."""
     import proteinsim2015_amd as ps
     if name == "c1":
@@ -271,7 +274,8 @@ def small_leg(name: str, steps: int, warmup: int = 10) -> dict:
     with ps.GpuSystem(geom, mods, integ) as sysg:
         sysg.set_state(pos, vel=vel)
         # steps of a few tens of microseconds: replayed as hipGraphs of 50
-        sysg.set_graph(50)
+        if not eager:
+            sysg.set_graph(50)
         sysg.step_device_noise(max(warmup, 50), 0)
         sysg.sync()
         t0 = time.perf_counter()
@@ -294,7 +298,7 @@ def small_leg(name: str, steps: int, warmup: int = 10) -> dict:
     pair_ms = kt["force_ms"] / max(1, kt["force_launches"])
     return {"workload": wl, "n_particles": n, "steps": steps, "kernel_path": info["path"],
             "ms_per_step": el / steps * 1e3, "particle_steps_per_s": value,
-            "launch": "hipGraph replays of 50 steps",
+            "launch": "eager launches" if eager else "hipGraph replays of 50 steps",
             "pair_kernel_ms": pair_ms,
             "g_and_b_kernel_ms": kt["integrate_ms"] / max(1, kt["integrate_launches"])
             if kt["integrate_launches"] else 0.0,
@@ -376,7 +380,7 @@ def main():
     small = {}
     if not args.no_small:
         for name, steps in (("c1", 1000), ("c2", 10000)):
-            small[name] = small_leg(name, steps)
+            small[name] = small_leg(name, steps, eager=args.eager)
             print(f"bench: {name} {small[name]['particle_steps_per_s']:.3g} particle-steps/s "
                   f"({small[name]['ms_per_step']:.4f} ms per step)", file=sys.stderr,
                   flush=True)

@@ -43,7 +43,9 @@ def initial_states(reps, seed: int = 1000):
 
 
 def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps: int = 50,
-        warmup: int = 5, verbose: bool = False) -> dict:
+        warmup: int = 5, verbose: bool = False, eager: bool = False) -> dict:
+    """eager: no hipGraph replays anywhere (the batched steps launched one by
+    one, the per-replica graph baseline skipped), e.g. under a profiler."""
     integ = E.BrownianIntegrator(kT=1.0, mass=1.0, gamma=1.0, dt=5e-4, vel_freq=1000, seed=90210)
     t0 = time.perf_counter()
     reps = make_replicas(nrep)
@@ -56,7 +58,8 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
     with E.Ensemble(reps, integ) as ens:
         ens.set_state(np.concatenate(pos), np.concatenate(vel))
         kernel, tile, _ = ens.autotune(3)
-        ens.set_graph(graph_steps)
+        if not eager:
+            ens.set_graph(graph_steps)
         ens.step(warmup)
         ens.sync()
         t1 = time.perf_counter()
@@ -85,24 +88,26 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
     for s in systems:
         s.sync()
     t4 = time.perf_counter()
-    for s in systems:
-        s.set_graph(baseline_steps)
-        s.step_device_noise(baseline_steps, 0)    # captures, then replays once
-    for s in systems:
-        s.sync()
-    t5 = time.perf_counter()
-    for s in systems:
-        s.step_device_noise(baseline_steps, 0)
-    for s in systems:
-        s.sync()
-    t6 = time.perf_counter()
+    t5 = t6 = None
+    if not eager:
+        for s in systems:
+            s.set_graph(baseline_steps)
+            s.step_device_noise(baseline_steps, 0)    # captures, then replays once
+        for s in systems:
+            s.sync()
+        t5 = time.perf_counter()
+        for s in systems:
+            s.step_device_noise(baseline_steps, 0)
+        for s in systems:
+            s.sync()
+        t6 = time.perf_counter()
     codes = [s.last_error()[0] for s in systems]
     for s in systems:
         s.close()
     if any(codes):
         raise RuntimeError(f"per-replica physics error {max(codes)}")
     single = ntot * baseline_steps / (t4 - t3)
-    single_graph = ntot * baseline_steps / (t6 - t5)
+    single_graph = ntot * baseline_steps / (t6 - t5) if t6 else None
     return {
         "replicas": nrep, "particles": ntot, "steps": steps,
         "batched_particle_steps_per_s": batched,
@@ -111,12 +116,12 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
         "per_replica_ms_per_step": (t4 - t3) / baseline_steps * 1e3,
         "per_replica_steps_timed": baseline_steps,
         "per_replica_graph_particle_steps_per_s": single_graph,
-        "per_replica_graph_ms_per_step": (t6 - t5) / baseline_steps * 1e3,
+        "per_replica_graph_ms_per_step": (t6 - t5) / baseline_steps * 1e3 if t6 else None,
         "speedup": batched / single,
-        "speedup_vs_graph_baseline": batched / single_graph,
+        "speedup_vs_graph_baseline": batched / single_graph if single_graph else None,
         "autotuned_kernel": kernel,
         "autotuned_tile": tile,
-        "graph_steps": graph_steps,
+        "graph_steps": None if eager else graph_steps,
     }
 
 
@@ -125,8 +130,11 @@ def main():
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--baseline-steps", type=int, default=20)
+    ap.add_argument("--eager", action="store_true",
+                    help="no hipGraph replays (for tracing under a profiler)")
     args = ap.parse_args()
-    print(json.dumps(run(args.replicas, args.steps, args.baseline_steps, verbose=True)))
+    print(json.dumps(run(args.replicas, args.steps, args.baseline_steps, verbose=True,
+                         eager=args.eager)))
 
 
 if __name__ == "__main__":

@@ -7,7 +7,7 @@
 // are spread over G = kDenseThreads / n_i lane groups; lane (i, g) culls
 // candidates g, g+G, ... of the staged list in batches of up to 32, one bit
 // per candidate in a register mask, then evaluates the pair forces of the
-// batch's survivors in candidate order.  Group partials are summed through
+// batch's survivors (last candidate first).  Group partials are summed through
 // LDS, then lanes 0..n_i-1 hand the home particles' forces to the epilogue.
 //
 // Neighbour sets are bit-exact with the reference: fast r^2 on the shifted
@@ -111,20 +111,20 @@ __device__ __forceinline__ void dense_pair(const Geo &g, const Pot &p,
     fz = fma(fr, dz, fz);
 }
 
-// a * b + c for a, b < 2^24 with b uniform over the wave (v_mad_u32_u24:
+// a + b * c for |b|, |c| < 2^23 with c uniform over the wave (v_mad_i32_i24:
 // one full-rate instruction; the compiler otherwise folds the index
 // arithmetic into a 64-bit mad)
-__device__ __forceinline__ int mad_u24(int a, int b_uniform, int c)
+__device__ __forceinline__ int mad_i24(int b, int c_uniform, int a)
 {
     int r;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b_uniform), "v"(c));
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(b), "s"(c_uniform), "v"(a));
     return r;
 }
 
 // Survivors of one batch of n <= 32 candidates j0, j0 + G, ...: bit
-// (n - 1 - k) of m is set when candidate j0 + k G passed the cull, so taking
-// the highest bit first keeps the candidate order (the bit with c leading
-// zeros is candidate k = c + n - 32).
+// (n - 1 - k) of m is set when candidate j0 + k G passed the cull.  The
+// lowest set bit is taken first (m &= m - 1), so the batch is evaluated
+// from its last candidate down: bit c is candidate j0 + (n - 1 - c) G.
 template <int MASK, int T, int LJN, class ST>
 __device__ void dense_drain_mask(const Geo &g, const Pot &p, const DenseSharedT<T> &sh,
                                  const double *__restrict__ pos, double xi, double yi,
@@ -132,11 +132,11 @@ __device__ void dense_drain_mask(const Geo &g, const Pot &p, const DenseSharedT<
                                  int n, int j0, int G, double &fx, double &fy, double &fz,
                                  DevErr *err, int id_i, const int *__restrict__ id, ST &st)
 {
-    const int jc = j0 + (n - 32) * G;
+    const int jtop = j0 + (n - 1) * G;
     while (m) {
-        const int c = __clz(m);
-        m ^= 0x80000000u >> c;
-        const int j = mad_u24(c, G, jc);
+        const int c = __builtin_ctz(m);
+        m &= m - 1u;
+        const int j = mad_i24(c, -G, jtop);
         if (j == self) continue;
         dense_pair<MASK, T, LJN>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, j, fx, fy, fz, err,
                                  id_i, id, st);

@@ -270,7 +270,7 @@ static __device__ const double kExp2Tab[64] = {
 // of it): -a = k ln2/64 - f with k = rint(-a 64/ln2) = 64 e + j (e = k >> 6,
 // j = k & 63, so 0 <= j < 64) and |f| <= ln2/128 (ln2/64 in two parts, from
 // the hi/lo split of ln 2 in fdlibm's e_exp.c); e^{-f} by its Taylor series
-// to degree 6 (truncation < 3e-20 relative), times 2^{j/64} from the table,
+// to degree 5 (truncation < 4e-17 relative, a third of an ulp), times 2^{j/64} from the table,
 // scaled by 2^e.  (2^{j/64} = 2 * 2^{-(64-j)/64} exactly in doubles, so this
 // rounds exactly as the same scheme with a table of 2^{-j/64} would.)
 __device__ __forceinline__ double exp_neg_tab(double a, const double *tab)
@@ -282,8 +282,7 @@ __device__ __forceinline__ double exp_neg_tab(double a, const double *tab)
     double f = fma(k, ln2_hi, a);
     f = fma(k, ln2_lo, f);
     const double x = -f;
-    double q = 1.0 / 720.0;
-    q = fma(q, x, 1.0 / 120.0);
+    double q = 1.0 / 120.0;
     q = fma(q, x, 1.0 / 24.0);
     q = fma(q, x, 1.0 / 6.0);
     q = fma(q, x, 0.5);

@@ -52,15 +52,16 @@ def test_table_exponential_within_two_ulp():
     inv = float(re.search(r"inv = ([0-9.eE+-]+);", fn).group(1))
     hi = float(re.search(r"ln2_hi = ([0-9.eE+-]+) / 64\.0;", fn).group(1)) / 64.0
     lo = float(re.search(r"ln2_lo = ([0-9.eE+-]+) / 64\.0;", fn).group(1)) / 64.0
+    q0 = 1.0 / float(re.search(r"double q = 1\.0 / ([0-9.]+);", fn).group(1))
     coef = [1.0 / float(d) for d in re.findall(r"fma\(q, x, 1\.0 / ([0-9.]+)\)", fn)]
-    assert len(coef) == 3 and "q = 1.0 / 720.0;" in fn
+    assert [round(1.0 / c) for c in [q0] + coef] == [120, 24, 6], "degree-5 Taylor series"
     coef += [0.5, 1.0, 1.0]
 
     def exp_neg_tab(a):
         k = float(np.rint(a * inv))
         f = _fma(k, lo, _fma(k, hi, a))
         x = -f
-        q = 1.0 / 720.0
+        q = q0
         for c in coef:
             q = _fma(q, x, c)
         ki = int(k)
