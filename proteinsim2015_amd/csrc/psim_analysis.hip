@@ -417,8 +417,7 @@ int build_hist_grid(psim_ctx *c, double r_lo, double width, int nbins, double si
     hg.c0 = 0;
     hipStream_t st = c->stream;
     k_hash_count<<<blocks_for(n, 256), 256, 0, st>>>(hg, c->pos, g.key, g.rank, g.count);
-    k_scan_reduce<<<g.nscan, kScanThreads, 0, st>>>(g.count, ncell, g.bsum);
-    k_scan_apply<<<g.nscan, kScanThreads, 0, st>>>(g.count, ncell, g.bsum, g.cstart, n);
+    launch_scan3(st, g.count, ncell, g.nscan, g.bsum, g.cstart, n);
     k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, g.key, g.rank, g.cstart, g.perm + n,
                                                   nullptr, 0);
     k_hg_canon<<<blocks_for(n, 256), 256, 0, st>>>(n, g.perm + n, g.key, g.cstart, g.perm);

@@ -62,48 +62,94 @@ __device__ int block_exclusive_scan(int v, int *total)
     return pre;
 }
 
+// The counts of one tile (kScanItems consecutive cells per thread, one
+// 16-byte load where the tile is whole).
+__device__ __forceinline__ void tile_counts(const int *count, int ncell, int base,
+                                            int (&c)[kScanItems])
+{
+    static_assert(kScanItems == 4, "one int4 per thread");
+    if (base + kScanItems <= ncell) {
+        const int4 q = *reinterpret_cast<const int4 *>(count + base);
+        c[0] = q.x;
+        c[1] = q.y;
+        c[2] = q.z;
+        c[3] = q.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) c[k] = (base + k < ncell) ? count[base + k] : 0;
+    }
+}
+
+// Tile totals: bsum[b] = sum of the counts of tile b.
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int *count, int ncell,
                                                               int *bsum)
 {
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    int v = 0;
+    int c[kScanItems];
+    tile_counts(count, ncell, blockIdx.x * kScanTile + threadIdx.x * kScanItems, c);
+    int v = c[0] + c[1] + c[2] + c[3];
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < ncell) v += count[base + k];
-    int total;
-    block_exclusive_scan(v, &total);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    __shared__ int wsum[kScanThreads / 64];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) t += wsum[w];
+        bsum[blockIdx.x] = t;
+    }
 }
 
-// Cell start table from the counts: each block adds the totals of the tiles
-// before it (bsum, written by k_scan_reduce) itself, then scans its tile.
+// The tile totals replaced by their exclusive prefix sums (one block).
+__global__ __launch_bounds__(kScanThreads) void k_scan_bsum(int *bsum, int nb)
+{
+    int carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += kScanThreads) {
+        const int i = b0 + (int)threadIdx.x;
+        const int v = i < nb ? bsum[i] : 0;
+        int total;
+        const int pre = block_exclusive_scan(v, &total);
+        if (i < nb) bsum[i] = carry + pre;
+        carry += total;
+    }
+}
+
+// Cell start table from the counts: block b scans its tile on top of the
+// prefix bsum[b] (k_scan_reduce, then k_scan_bsum), writes the starts and
+// clears the counts for the next step's counting.
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(int *count, int ncell,
                                                              const int *bsum,
                                                              int *cstart, int n)
 {
-    int before = 0;
-    for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) before += bsum[b];
-    int before_total;
-    block_exclusive_scan(before, &before_total);
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int c[kScanItems];
-    int v = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        c[k] = (base + k < ncell) ? count[base + k] : 0;
-        v += c[k];
-    }
+    tile_counts(count, ncell, base, c);
     int total;
-    int pre = block_exclusive_scan(v, &total) + before_total;
+    int pre = block_exclusive_scan(c[0] + c[1] + c[2] + c[3], &total) + bsum[blockIdx.x];
+    if (base + kScanItems <= ncell) {
+        const int4 st = make_int4(pre, pre + c[0], pre + c[0] + c[1], pre + c[0] + c[1] + c[2]);
+        *reinterpret_cast<int4 *>(cstart + base) = st;
+        *reinterpret_cast<int4 *>(count + base) = make_int4(0, 0, 0, 0);
+    } else {
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < ncell) {
-            cstart[base + k] = pre;
-            count[base + k] = 0;
+        for (int k = 0; k < kScanItems; ++k) {
+            if (base + k < ncell) {
+                cstart[base + k] = pre;
+                count[base + k] = 0;
+            }
+            pre += c[k];
         }
-        pre += c[k];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
+}
+
+// The three-launch scan of the cell counts into the start table (grids of
+// more than kScanDirectTiles tiles).
+inline void launch_scan3(hipStream_t st, int *count, int ncell, int nscan, int *bsum,
+                         int *cstart, int n)
+{
+    k_scan_reduce<<<nscan, kScanThreads, 0, st>>>(count, ncell, bsum);
+    k_scan_bsum<<<1, kScanThreads, 0, st>>>(bsum, nscan);
+    k_scan_apply<<<nscan, kScanThreads, 0, st>>>(count, ncell, bsum, cstart, n);
 }
 
 // Cell start table in one launch for grids of at most kScanDirectTiles
@@ -205,12 +251,17 @@ __global__ void k_move(int n, const int *__restrict__ perm, const int *__restric
 // integrator reads the previous positions, forces and memory of the last
 // step where it wrote them (at prev[s]) instead of every array moving every
 // step.
+// With ctl set (a step sequence whose control lives in device memory), the
+// kernel also ends the step: it advances the clock and the next step's
+// flags, which no kernel of this step reads after the integrator.
 __global__ void k_move_lagged(int n, const int *__restrict__ perm, const int *__restrict__ key,
                               const int *__restrict__ cstart, const double *__restrict__ npos,
                               const int *__restrict__ id_in, double *__restrict__ pos,
-                              int *__restrict__ id_out, int *__restrict__ prev)
+                              int *__restrict__ id_out, int *__restrict__ prev,
+                              StepCtl *ctl)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ctl && t == 0) ctl_advance(ctl);
     if (t >= n) return;
     const int s = perm[t];
     const int c = key[s];

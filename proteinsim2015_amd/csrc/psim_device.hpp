@@ -68,8 +68,8 @@ struct GB {
 };
 
 // Step control kept in device memory so that a captured sequence of steps
-// can be replayed: the step kernels read it, k_ctl_advance at the end of each
-// step advances it.
+// can be replayed: the step kernels read it, the last kernel of each step
+// (the cell rebuild's move) advances it.
 // (brownianIntegrator.cpp:183 first-step test, :288-292 velocity counter,
 // system.cpp:210 time advance)
 struct StepCtl {
@@ -81,6 +81,27 @@ struct StepCtl {
     int first;                 // currentTime == 0
     int do_vel;                // velocityStep this step
 };
+
+// Flags of the step about to run from its clock and velocity counter
+// (brownianIntegrator.cpp:183 first step, :288-292 velocity counter).
+__device__ __forceinline__ void ctl_prepare(StepCtl *c)
+{
+    c->first = (c->time == 0) ? 1 : 0;
+    c->do_vel = 0;
+    if (!c->first) {
+        c->do_vel = (c->vel_freq == 0 || c->vel_counter == c->vel_freq) ? 1 : 0;
+        if (c->vel_counter == c->vel_freq) c->vel_counter = 0; else c->vel_counter++;
+    }
+}
+
+// End of step: advance the clock and set the flags of the next step
+// (system.cpp:210).
+__device__ __forceinline__ void ctl_advance(StepCtl *c)
+{
+    c->step += 1;
+    c->time += c->dt;
+    ctl_prepare(c);
+}
 
 struct DevErr {
     int code;

@@ -15,13 +15,15 @@
 //               velocityStep), wrap (particle::setPos), new cell hash and
 //               an atomic per-cell count; writes the next state in the
 //               current slot order to the "n*" buffers.
-//   k_scan_*    counts -> cstart (exclusive scan), counts cleared (one
-//               launch, k_scan_direct, for grids of up to 32 scan tiles;
-//               k_scatter then clears the counts).
+//   k_scan_*    counts -> cstart (exclusive scan), counts cleared: tile
+//               totals, their prefix (one block), then each tile's scan;
+//               one launch, k_scan_direct, for grids of up to 32 scan
+//               tiles (k_scatter then clears the counts).
 //   k_scatter   slot of each particle in its new cell (arbitrary order).
-//   k_move      canonical within-cell order (ascending previous slot, i.e.
-//               a stable counting sort) and the state gather into the
-//               sorted arrays.
+//   k_move_lagged  canonical within-cell order (ascending previous slot,
+//               i.e. a stable counting sort); positions and ids move into
+//               it, the previous positions, forces and memory stay where
+//               the step wrote them (read through the slot map prev).
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -84,7 +86,7 @@ int collect_times(psim_ctx *c)
 }
 
 // counts (already accumulated) -> cstart, scatter, canonical move.
-int launch_build(psim_ctx *c)
+int launch_build(psim_ctx *c, StepCtl *advance)
 {
     const int n = c->n, nc = c->ncell;
     hipStream_t st = c->stream;
@@ -92,8 +94,7 @@ int launch_build(psim_ctx *c)
     if (direct) {
         k_scan_direct<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->cstart, n);
     } else {
-        k_scan_reduce<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->bsum);
-        k_scan_apply<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->bsum, c->cstart, n);
+        launch_scan3(st, c->count, nc, c->nscan, c->bsum, c->cstart, n);
     }
     k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, c->key, c->arank, c->cstart, c->perm,
                                                   direct ? c->count : nullptr, nc);
@@ -101,7 +102,8 @@ int launch_build(psim_ctx *c)
     // forces and memory stay where this step wrote them (npos0 / nfrc /
     // nmem, old order) and the next step reads them through prev
     k_move_lagged<<<blocks_for(n, 256), 256, 0, st>>>(n, c->perm, c->key, c->cstart, c->npos,
-                                                      c->id, c->pos, c->nid, c->prev);
+                                                      c->id, c->pos, c->nid, c->prev,
+                                                      advance);
     CK(hipGetLastError());
     std::swap(c->id, c->nid);
     std::swap(c->pos0, c->npos0);
@@ -277,7 +279,7 @@ void next_flags(psim_ctx *c, int *first, int *do_vel)
 
 // One step's kernels: forces (+ G&B), separate G&B on the dense path, cell
 // rebuild.
-int enqueue_step(psim_ctx *c, const StepArgs &a, psim_ctx::Mark *mk)
+int enqueue_step(psim_ctx *c, const StepArgs &a, psim_ctx::Mark *mk, StepCtl *advance = nullptr)
 {
     if (mk) {
         hipEvent_t e = take_event(c, &mk->e0);
@@ -297,7 +299,7 @@ int enqueue_step(psim_ctx *c, const StepArgs &a, psim_ctx::Mark *mk)
         hipEvent_t e = take_event(c, &mk->e1b);
         if (e) CK(hipEventRecord(e, c->stream));
     }
-    rc = launch_build(c);
+    rc = launch_build(c, advance);
     if (rc) return rc;
     if (mk) {
         hipEvent_t e = take_event(c, &mk->e2);
@@ -345,7 +347,8 @@ int replay_graph(psim_ctx *c)
     }
     const GraphKey key = graph_key(c);
     if (c->gexec && std::memcmp(&key, &c->gkey, sizeof key) != 0) drop_graph(c);
-    // step control of the first captured step; k_ctl_advance prepares the rest
+    // step control of the first captured step; each step's last kernel
+    // prepares the next
     StepCtl h{};
     h.step = c->step_index;
     h.time = c->time;
@@ -359,8 +362,7 @@ int replay_graph(psim_ctx *c)
         int rc = 0;
         for (int k = 0; k < c->graph_steps && !rc; ++k) {
             const StepArgs a = step_args(c, nullptr, c->ctl);
-            rc = enqueue_step(c, a, nullptr);
-            if (!rc) k_ctl_advance<<<1, 64, 0, c->stream>>>(c->ctl);
+            rc = enqueue_step(c, a, nullptr, c->ctl);
         }
         hipGraph_t g = nullptr;
         const hipError_t ce = hipStreamEndCapture(c->stream, &g);
@@ -512,7 +514,7 @@ int set_state(psim_ctx *c, const double *pos, const double *pos0, const double *
     return 0;
 }
 
-int launch_build(psim_ctx *c) { return ::launch_build(c); }
+int launch_build(psim_ctx *c) { return ::launch_build(c, nullptr); }
 int settle(psim_ctx *c) { return ::settle(c); }
 int run_steps(psim_ctx *c, int nsteps, const double *d_noise)
 {

@@ -311,7 +311,7 @@ void launch_ens_pair_block(psim_ens *e, const StepArgs &a)
     }
 }
 
-void launch_build_ens(psim_ens *e)
+void launch_build_ens(psim_ens *e, StepCtl *advance)
 {
     const int n = e->ntot, nc = e->ncell;
     hipStream_t st = e->stream;
@@ -319,15 +319,14 @@ void launch_build_ens(psim_ens *e)
     if (direct) {
         k_scan_direct<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->cstart, n);
     } else {
-        k_scan_reduce<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->bsum);
-        k_scan_apply<<<e->nscan, kScanThreads, 0, st>>>(e->count, nc, e->bsum, e->cstart, n);
+        launch_scan3(st, e->count, nc, e->nscan, e->bsum, e->cstart, n);
     }
     k_scatter<<<eblocks(n, 256), 256, 0, st>>>(n, e->key, e->arank, e->cstart, e->perm,
                                                direct ? e->count : nullptr, nc);
     // positions and ids move into the new order; the previous positions,
     // forces and memory stay where this step wrote them, read through prev
     k_move_lagged<<<eblocks(n, 256), 256, 0, st>>>(n, e->perm, e->key, e->cstart, e->npos,
-                                                   e->id, e->pos, e->nid, e->prev);
+                                                   e->id, e->pos, e->nid, e->prev, advance);
     std::swap(e->id, e->nid);
     std::swap(e->pos0, e->npos0);
     std::swap(e->frc, e->nfrc);
@@ -384,8 +383,7 @@ int enqueue_step(psim_ens *e)
     case 7: launch_ens_pair_block<7>(e, a); break;
     default: return PSIM_E_ARG;
     }
-    launch_build_ens(e);
-    k_ctl_advance<<<1, 64, 0, e->stream>>>(e->ctl);
+    launch_build_ens(e, e->ctl);   // its last kernel also advances the step control
     ECK(hipGetLastError());
     return 0;
 }
@@ -638,7 +636,7 @@ int psim_ens_set_state(psim_ens *e, const double *pos, const double *vel, double
     ECK(hipMemcpy(e->ctl, &c, sizeof(c), hipMemcpyHostToDevice));
     k_ens_hash<<<eblocks(n, 256), 256, 0, e->stream>>>(e->reps, e->p0, e->nrep, n, e->npos,
                                                        e->id, e->key, e->arank, e->count);
-    launch_build_ens(e);
+    launch_build_ens(e, nullptr);
     ECK(hipGetLastError());
     ECK(hipStreamSynchronize(e->stream));
     e->have_state = true;

@@ -202,8 +202,7 @@ extern "C" int psim_gpu_forces_noalloc(const psim_params *params, const double *
     if (direct) {
         k_scan_direct<<<nscan, kScanThreads, 0, st>>>(count, ncell, cstart, n);
     } else {
-        k_scan_reduce<<<nscan, kScanThreads, 0, st>>>(count, ncell, bsum);
-        k_scan_apply<<<nscan, kScanThreads, 0, st>>>(count, ncell, bsum, cstart, n);
+        launch_scan3(st, count, ncell, nscan, bsum, cstart, n);
     }
     k_scatter<<<eng::blocks_for(n, 256), 256, 0, st>>>(n, key, rank, cstart, perm,
                                                        direct ? count : nullptr, ncell);

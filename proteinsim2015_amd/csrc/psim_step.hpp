@@ -67,26 +67,6 @@ inline PathChoice choose_path(const psim_params &P, const Geo &g)
     return c;
 }
 
-__device__ __forceinline__ void ctl_prepare(StepCtl *c)
-{
-    c->first = (c->time == 0) ? 1 : 0;
-    c->do_vel = 0;
-    if (!c->first) {
-        c->do_vel = (c->vel_freq == 0 || c->vel_counter == c->vel_freq) ? 1 : 0;
-        if (c->vel_counter == c->vel_freq) c->vel_counter = 0; else c->vel_counter++;
-    }
-}
-
-// End of step: advance the clock and set the flags of the next step
-// (system.cpp:210; brownianIntegrator.cpp:183, 288-292).
-__global__ void k_ctl_advance(StepCtl *c)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        c->step += 1;
-        c->time += c->dt;
-        ctl_prepare(c);
-    }
-}
 
 struct StepArgs {
     const double *pos, *pos0, *frc, *mem;
