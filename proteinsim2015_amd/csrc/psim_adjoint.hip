@@ -169,11 +169,16 @@ __device__ void visit_pairs(const Geo &g, const Pot &p, const double *pos, const
 }
 
 // One backward step at the recorded state x_n (SPLIT lanes per particle).
+// lam (lambda_n) is in the slot order of that state, so a neighbour's value
+// is read at its slot; lambda_{n-1} goes to lam_prev in the slot order of
+// the state before (inv: original index -> slot there) or, with inv null, by
+// original index.  The a_n are kept by original index.
 template <int MASK, bool EXACT, int BLOCK, int SPLIT>
 __global__ __launch_bounds__(BLOCK) void k_adj_step(
     Geo g, Pot p, AdjPot q, double radius, const double *pos, const int *id,
     const int *cstart, const double *lam, const double *a1, const double *a2, double *a0,
-    double *lam_prev, double beta, double coeff0, double alpha_prev, double c2, double *tpart)
+    double *lam_prev, const int *inv, double beta, double coeff0, double alpha_prev,
+    double c2, double *tpart)
 {
     const int gt = xcd_block() * BLOCK + threadIdx.x;
     const int s = gt / SPLIT, sub = gt % SPLIT;
@@ -181,14 +186,12 @@ __global__ __launch_bounds__(BLOCK) void k_adj_step(
     double th[kNTheta] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double jx = 0.0, jy = 0.0, jz = 0.0;
     if (s < N) {   // whole lane groups
-        const int o = id[s];
-        const double lx = lam[o], ly = lam[N + o], lz = lam[2 * N + o];
+        const double lx = lam[s], ly = lam[N + s], lz = lam[2 * N + s];
         const double size = 2.0 * radius;
         visit_pairs<EXACT, SPLIT>(g, p, pos, cstart, s, sub,
                                   [&](int j, double dx, double dy, double dz, double r,
                                       double rinv) {
-            const int oj = id[j];
-            const double ex = lam[oj] - lx, ey = lam[N + oj] - ly, ez = lam[2 * N + oj] - lz;
+            const double ex = lam[j] - lx, ey = lam[N + j] - ly, ez = lam[2 * N + j] - lz;
             double f, fp, dth[kNTheta];
             pair_fnet_d<MASK>(p, q, r, rinv, size, f, fp, dth);
             const double gr = f * rinv;
@@ -218,9 +221,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_step(
         a0[N + o] = n0y;
         a0[2 * N + o] = n0z;
         if (lam_prev) {
-            lam_prev[o] = alpha_prev * n0x - c2 * a1[o];
-            lam_prev[N + o] = alpha_prev * n0y - c2 * a1[N + o];
-            lam_prev[2 * N + o] = alpha_prev * n0z - c2 * a1[2 * N + o];
+            const int w = inv ? inv[o] : o;
+            lam_prev[w] = alpha_prev * n0x - c2 * a1[o];
+            lam_prev[N + w] = alpha_prev * n0y - c2 * a1[N + o];
+            lam_prev[2 * N + w] = alpha_prev * n0z - c2 * a1[2 * N + o];
         }
     }
     __shared__ double red[kNTheta][BLOCK / 64];
@@ -259,6 +263,27 @@ __global__ __launch_bounds__(256) void k_theta_sum(const double *tpart, long lon
     if (t == 0) acc[k] = red[0];
 }
 
+// First stage of the dL/dtheta sum: block (c, k) sums the block partials
+// b of chunk c (b = c, c + kThetaChunks, ...) of parameter k into
+// part[c][k], which k_theta_sum then sums in chunk order (deterministic).
+constexpr int kThetaChunks = 256;
+__global__ __launch_bounds__(256) void k_theta_chunks(const double *tpart, long long nb,
+                                                      double *part)
+{
+    __shared__ double red[256];
+    const int c = blockIdx.x, k = blockIdx.y, t = threadIdx.x;
+    double v = 0.0;
+    for (long long b = c + (long long)kThetaChunks * t; b < nb; b += (long long)kThetaChunks * 256)
+        v += tpart[(size_t)b * kNTheta + k];
+    red[t] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) part[(size_t)c * kNTheta + k] = red[0];
+}
+
 // Several device-to-device copies in one launch (the state arrays of a
 // checkpoint or of a recorded step): segment k copies words[k] 4-byte words.
 constexpr int kMaxSeg = 8;
@@ -279,6 +304,23 @@ __global__ __launch_bounds__(256) void k_copy_segs(CopySegs cs)
         const long long off = w - (k ? cs.end[k - 1] : 0);
         cs.dst[k][off] = cs.src[k][off];
     }
+}
+
+// inv[id[s]] = s: the slot of each original index
+__global__ void k_inv(int n, const int *id, int *inv)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) inv[id[s]] = s;
+}
+
+// out[d][s] = in[d][id[s]]: a vector by original index into slot order
+__global__ void k_gather_slots(int n, const int *id, const double *in, double *out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int o = id[s];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[(size_t)d * n + s] = in[(size_t)d * n + o];
 }
 
 // lam = alpha * a1 (the last step: a_{K+1} = 0)
@@ -303,6 +345,7 @@ struct psim_adjoint {
     int *cp_i = nullptr;           // (ncp + 1) x (N + ncell + 1): id, cstart
     double *rec_pos = nullptr;     // every x 3N
     int *rec_i = nullptr;          // every x (N + ncell + 1)
+    int *rec_inv = nullptr;        // every x N: original index -> slot of each record
     double *a[3] = {nullptr, nullptr, nullptr};
     double *lam[2] = {nullptr, nullptr};
     double *tpart = nullptr, *acc = nullptr;
@@ -426,14 +469,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_stats(Geo g, Pot p, const double 
 template <int MASK>
 void launch_adj_mask(psim_adjoint *A, const AdjPot &q, const double *pos, const int *id,
                      const int *cs, const double *lam, const double *a1, const double *a2,
-                     double *a0, double *lam_prev, double beta, double alpha_prev, double *tp)
+                     double *a0, double *lam_prev, const int *inv, double beta,
+                     double alpha_prev, double *tp)
 {
     psim_ctx *c = A->c;
     const unsigned nb = adj_blocks(c);
 #define PSIM_ADJ_LAUNCH(EX, BL, SP)                                                        \
     k_adj_step<MASK, EX, BL, SP><<<nb, BL, 0, c->stream>>>(                                \
-        c->geo, c->pot, q, c->prm.radius, pos, id, cs, lam, a1, a2, a0, lam_prev, beta,    \
-        c->gb.coeff0, alpha_prev, c->gb.c2, tp)
+        c->geo, c->pot, q, c->prm.radius, pos, id, cs, lam, a1, a2, a0, lam_prev, inv,     \
+        beta, c->gb.coeff0, alpha_prev, c->gb.c2, tp)
     const bool split = adj_split(c) > 1;
     if (c->geo.exact_mi) {
         if (split) PSIM_ADJ_LAUNCH(true, kAdjBlock, PSIM_ADJ_SPLIT);
@@ -492,10 +536,11 @@ int psim_adj_create(psim_ctx *c, int nsteps, int checkpoint_every, psim_adjoint 
     if (!rc) rc = dalloc(&A->cp_i, (size_t)(A->ncp + 1) * cp_stride_i(c));
     if (!rc) rc = dalloc(&A->rec_pos, (size_t)A->every * n3);
     if (!rc) rc = dalloc(&A->rec_i, (size_t)A->every * cp_stride_i(c));
+    if (!rc) rc = dalloc(&A->rec_inv, (size_t)A->every * c->n);
     for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(&A->a[k], n3);
     for (int k = 0; k < 2 && !rc; ++k) rc = dalloc(&A->lam[k], n3);
     if (!rc) rc = dalloc(&A->tpart, (size_t)nsteps * nb * kNTheta);
-    if (!rc) rc = dalloc(&A->acc, kNTheta);
+    if (!rc) rc = dalloc(&A->acc, kNTheta * (1 + (size_t)kThetaChunks));
     if (!rc) rc = save_cp(A, 0);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = PSIM_E_HIP;
     if (rc) {
@@ -510,7 +555,7 @@ void psim_adj_destroy(psim_adjoint *A)
 {
     if (!A) return;
     if (A->c && A->c->stream) (void)hipStreamSynchronize(A->c->stream);
-    void *bufs[] = {A->cp_d, A->cp_i, A->rec_pos, A->rec_i, A->a[0], A->a[1], A->a[2],
+    void *bufs[] = {A->cp_d, A->cp_i, A->rec_pos, A->rec_i, A->rec_inv, A->a[0], A->a[1], A->a[2],
                     A->lam[0], A->lam[1], A->tpart, A->acc};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -589,31 +634,50 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
         const int n0 = k * A->every;
         const int n1 = n0 + A->every < A->K ? n0 + A->every : A->K;
         if ((rc = restore_cp(A, k))) return rc;
-        for (int s = n0; s < n1; ++s) {
-            const int r = s - n0;
+        {
+            // the checkpoint's state is record 0; each recomputed step's
+            // cell rebuild writes the next record itself
             CopySegs cs{};
-            add_seg(cs, c->pos, A->rec_pos + (size_t)r * n3, sizeof(double) * n3);
-            add_seg(cs, c->id, A->rec_i + (size_t)r * si, sizeof(int) * n);
-            add_seg(cs, c->cstart, A->rec_i + (size_t)r * si + n, sizeof(int) * (c->ncell + 1));
+            add_seg(cs, c->pos, A->rec_pos, sizeof(double) * n3);
+            add_seg(cs, c->id, A->rec_i, sizeof(int) * n);
+            add_seg(cs, c->cstart, A->rec_i + n, sizeof(int) * (c->ncell + 1));
             if ((rc = copy_segs(c, cs))) return rc;
-            if (s + 1 < n1 && (rc = eng::run_steps(c, 1, nullptr))) return rc;
+            k_inv<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, A->rec_i, A->rec_inv);
         }
+        for (int s = n0; s + 1 < n1; ++s) {
+            const int r = s - n0 + 1;
+            c->rec.pos = A->rec_pos + (size_t)r * n3;
+            c->rec.id = A->rec_i + (size_t)r * si;
+            c->rec.cstart = A->rec_i + (size_t)r * si + n;
+            c->rec.inv = A->rec_inv + (size_t)r * n;
+            rc = eng::run_steps(c, 1, nullptr);
+            c->rec = psim_ctx::Record{};
+            if (rc) return rc;
+        }
+        // lambda_{n1-1} arrives by original index (from the segment after,
+        // or the start): into the slot order of the segment's last record
+        k_gather_slots<<<blocks_for(n, 256), 256, 0, c->stream>>>(
+            n, A->rec_i + (size_t)(n1 - 1 - n0) * si, lam, lam_prev);
+        std::swap(lam, lam_prev);
         for (int s = n1 - 1; s >= n0; --s) {
             const int r = s - n0;
             const double *pos = A->rec_pos + (size_t)r * n3;
             const int *id = A->rec_i + (size_t)r * si;
             const int *cs = id + n;
             double *lp = s >= 1 ? lam_prev : nullptr;
+            // lambda_{s-1} in the slot order of record r - 1 within the
+            // segment, by original index across its start
+            const int *inv = r >= 1 ? A->rec_inv + (size_t)(r - 1) * n : nullptr;
             const double ap = s >= 1 ? alpha(s - 1) : 0.0;
             double *tp = A->tpart + (size_t)s * nbk * kNTheta;
             switch (c->pot.mask) {
-            case 1: launch_adj_mask<1>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
-            case 2: launch_adj_mask<2>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
-            case 3: launch_adj_mask<3>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
-            case 4: launch_adj_mask<4>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
-            case 5: launch_adj_mask<5>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
-            case 6: launch_adj_mask<6>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
-            case 7: launch_adj_mask<7>(A, q, pos, id, cs, lam, a1, a2, a0, lp, beta(s), ap, tp); break;
+            case 1: launch_adj_mask<1>(A, q, pos, id, cs, lam, a1, a2, a0, lp, inv, beta(s), ap, tp); break;
+            case 2: launch_adj_mask<2>(A, q, pos, id, cs, lam, a1, a2, a0, lp, inv, beta(s), ap, tp); break;
+            case 3: launch_adj_mask<3>(A, q, pos, id, cs, lam, a1, a2, a0, lp, inv, beta(s), ap, tp); break;
+            case 4: launch_adj_mask<4>(A, q, pos, id, cs, lam, a1, a2, a0, lp, inv, beta(s), ap, tp); break;
+            case 5: launch_adj_mask<5>(A, q, pos, id, cs, lam, a1, a2, a0, lp, inv, beta(s), ap, tp); break;
+            case 6: launch_adj_mask<6>(A, q, pos, id, cs, lam, a1, a2, a0, lp, inv, beta(s), ap, tp); break;
+            case 7: launch_adj_mask<7>(A, q, pos, id, cs, lam, a1, a2, a0, lp, inv, beta(s), ap, tp); break;
             default: return PSIM_E_ARG;
             }
             CK(hipGetLastError());
@@ -627,7 +691,9 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
             lam_prev = t;
         }
     }
-    k_theta_sum<<<kNTheta, 256, 0, c->stream>>>(A->tpart, (long long)A->K * nbk, A->acc);
+    k_theta_chunks<<<dim3(kThetaChunks, kNTheta), 256, 0, c->stream>>>(
+        A->tpart, (long long)A->K * nbk, A->acc + kNTheta);
+    k_theta_sum<<<kNTheta, 256, 0, c->stream>>>(A->acc + kNTheta, kThetaChunks, A->acc);
     CK(hipGetLastError());
     // a1 now holds a_0 = dL/dx_0
     std::vector<double> h(gx0 ? n3 : 0);

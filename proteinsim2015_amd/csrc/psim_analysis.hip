@@ -23,6 +23,7 @@
 
 #include "psim_build.hpp"
 #include "psim_ctx.hpp"
+#include "psim_pairs.hpp"
 
 using namespace psim;
 using eng::blocks_for;
@@ -187,6 +188,7 @@ struct HistGeo {
     int S, S2;          // histogram grid (S >= 3), or S == 0 for all pairs
     double L, half, cell;
     double r_lo, width, sigma, inv_sigma, reach2;
+    double delta;       // width / sigma
     int nbins;
 };
 
@@ -315,8 +317,9 @@ __global__ __launch_bounds__(kAnaBlock) void k_pair_hist_grad(HistGeo h, const d
 
 // canonical within-cell order of the histogram grid (ascending slot), so the
 // neighbour visit order does not depend on atomic timing
+// (and the positions copied into that order, hpos [3][n])
 __global__ void k_hg_canon(int n, const int *perm_in, const int *key, const int *cstart,
-                           int *perm_out)
+                           int *perm_out, const double *pos, double *hpos)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -326,6 +329,207 @@ __global__ void k_hg_canon(int n, const int *perm_in, const int *key, const int 
     int rank = 0;
     for (int u = b; u < e; ++u) rank += (perm_in[u] < s);
     perm_out[b + rank] = s;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) hpos[(size_t)d * n + b + rank] = pos[(size_t)d * n + s];
+}
+
+// ---- pair histogram on the histogram grid (S >= 3) ---------------------------
+// SPLIT lanes per particle (grid order t): lane `sub` walks the x-runs sub,
+// sub + SPLIT, ... of the 27 grid cells (psim_pairs.hpp x_run) over the
+// grid-ordered positions; every j != t within reach is visited with the
+// reference's minimum-image distance.
+constexpr int kHistSplit = 8;
+
+template <int SPLIT, class Fn>
+__device__ __forceinline__ void visit_grid(const HistGeo &h, const double *__restrict__ hpos,
+                                           const int *__restrict__ cstart, int t, int sub,
+                                           double xi, double yi, double zi, Fn fn)
+{
+    const int N = h.n, S = h.S;
+    const int cx = cell_1d(xi, h.cell, S), cy = cell_1d(yi, h.cell, S),
+              cz = cell_1d(zi, h.cell, S);
+    const StartTab tab{cstart};
+    for (int q = sub; q < 9; q += SPLIT) {
+        int ry = cy + q / 3 - 1, rz = cz + q % 3 - 1;
+        ry = ry < 0 ? ry + S : (ry >= S ? ry - S : ry);
+        rz = rz < 0 ? rz + S : (rz >= S ? rz - S : rz);
+        const XRun xr = x_run(tab, S * ry + h.S2 * rz, cx, S);
+#pragma unroll
+        for (int sg = 0; sg < 2; ++sg) {
+            const int e = sg ? xr.e1 : xr.e0;
+            for (int u = sg ? xr.b1 : xr.b0; u < e; ++u) {
+                if (u == t) continue;
+                const double xj = hpos[u], yj = hpos[N + u], zj = hpos[2 * N + u];
+                const double r2 = pbc_dist_ref(xi, yi, zi, xj, yj, zj, h.L, h.half);
+                if (r2 < h.reach2) fn(xj, yj, zj, r2);
+            }
+        }
+    }
+}
+
+// Per-block partial histograms (smoothed, sigma > 0).  Each lane keeps the
+// r^2 of its pairs within reach in a short list in LDS and evaluates them
+// (the edge CDFs) when some lane of the wave has filled its list, and once
+// at the end, so that the costly part runs with most lanes busy.  Each wave
+// adds into its own copy of the histogram in LDS; the block's sum goes to
+// part[blockIdx.x][*], summed in a fixed order by k_hist_sum.
+constexpr int kHistCap = 8;
+template <int SPLIT>
+__global__ __launch_bounds__(kAnaBlock) void k_pair_hist_grid(HistGeo h,
+                                                              const double *__restrict__ hpos,
+                                                              const int *__restrict__ cstart,
+                                                              double *part)
+{
+    extern __shared__ double lds[];   // lists [kHistCap][kAnaBlock], then [4][nbins]
+    double *lst = lds;
+    double *wh = lds + kHistCap * kAnaBlock;
+    const int nb = h.nbins;
+    for (int b = threadIdx.x; b < (kAnaBlock / 64) * nb; b += kAnaBlock) wh[b] = 0.0;
+    __syncthreads();
+    double *mine = wh + (threadIdx.x >> 6) * nb;
+    const int gt = blockIdx.x * kAnaBlock + threadIdx.x;
+    const int t = gt / SPLIT, sub = gt % SPLIT;
+    auto drain = [&](int nh) {
+        for (int k = 0; k < nh; ++k) {
+            const double r = sqrt(lst[k * kAnaBlock + threadIdx.x]);
+            int k0, k1;
+            edge_window(h, r, k0, k1);
+            double prev = 0.0;   // CDF at the edge below
+            for (int q = k0; q <= k1; ++q) {
+                const double e = h.r_lo + h.width * q;
+                const double E = norm_cdf((e - r) * h.inv_sigma);
+                if (q - 1 >= 0 && q - 1 < nb) atomicAdd(&mine[q - 1], E - prev);
+                prev = E;
+            }
+            if (k1 < nb && k1 >= 0 && k0 <= k1) atomicAdd(&mine[k1], 1.0 - prev);
+            else if (k0 > k1 && k1 >= 0 && k1 < nb) atomicAdd(&mine[k1], 1.0);
+        }
+    };
+    int nh = 0;
+    if (t < h.n) {
+        const int N = h.n, S = h.S;
+        const double xi = hpos[t], yi = hpos[N + t], zi = hpos[2 * N + t];
+        const int cx = cell_1d(xi, h.cell, S), cy = cell_1d(yi, h.cell, S),
+                  cz = cell_1d(zi, h.cell, S);
+        const StartTab tab{cstart};
+        for (int q = sub; q < 9; q += SPLIT) {
+            int ry = cy + q / 3 - 1, rz = cz + q % 3 - 1;
+            ry = ry < 0 ? ry + S : (ry >= S ? ry - S : ry);
+            rz = rz < 0 ? rz + S : (rz >= S ? rz - S : rz);
+            const XRun xr = x_run(tab, S * ry + h.S2 * rz, cx, S);
+#pragma unroll
+            for (int sg = 0; sg < 2; ++sg) {
+                const int e = sg ? xr.e1 : xr.e0;
+                for (int u = sg ? xr.b1 : xr.b0; u < e; ++u) {
+                    if (u != t) {
+                        const double r2 = pbc_dist_ref(xi, yi, zi, hpos[u], hpos[N + u],
+                                                       hpos[2 * N + u], h.L, h.half);
+                        if (r2 < h.reach2) {
+                            lst[nh * kAnaBlock + threadIdx.x] = r2;
+                            ++nh;
+                        }
+                    }
+                    if (__any(nh == kHistCap)) {
+                        drain(nh);
+                        nh = 0;
+                    }
+                }
+            }
+        }
+    }
+    drain(nh);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += kAnaBlock) {
+        double v = 0.0;
+        for (int w = 0; w < kAnaBlock / 64; ++w) v += wh[w * nb + b];
+        part[(size_t)blockIdx.x * nb + b] = v;
+    }
+}
+
+// hist[b] = sum over the nparts block partials of bin b: block b, lane t
+// sums partials t, t + 256, ... in order, then a fixed tree (reproducible).
+__global__ __launch_bounds__(256) void k_hist_sum(const double *part, int nparts, int nb,
+                                                  double *hist)
+{
+    __shared__ double red[256];
+    const int b = blockIdx.x, t = threadIdx.x;
+    double v = 0.0;
+    for (int k = t; k < nparts; k += 256) v += part[(size_t)k * nb + b];
+    red[t] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) hist[b] = red[0];
+}
+
+// Gradient of sum_b w_b h_b on the histogram grid (as k_pair_hist_grad);
+// dw[k] = w_k - w_{k-1} in LDS, and along a pair's edge window the Gaussian
+// density by the recurrence phi(t + delta) = phi(t) e^{-delta t} e^{-delta^2/2}
+// (two exponentials per pair instead of one per edge) when delta <= 1.
+template <int SPLIT>
+__global__ __launch_bounds__(kAnaBlock) void k_pair_hist_grad_grid(
+    HistGeo h, const double *__restrict__ hpos, const int *__restrict__ perm,
+    const int *__restrict__ cstart, const int *__restrict__ id, const double *__restrict__ w,
+    double *grad)
+{
+    extern __shared__ double dw[];   // nbins + 1
+    const int nb = h.nbins;
+    for (int k = threadIdx.x; k <= nb; k += kAnaBlock)
+        dw[k] = (k < nb ? w[k] : 0.0) - (k >= 1 ? w[k - 1] : 0.0);
+    __syncthreads();
+    const int gt = blockIdx.x * kAnaBlock + threadIdx.x;
+    const int t = gt / SPLIT, sub = gt % SPLIT;
+    if (t >= h.n) return;   // whole lane groups
+    const int N = h.n;
+    const double xi = hpos[t], yi = hpos[N + t], zi = hpos[2 * N + t];
+    const bool recur = h.delta <= 1.0;
+    const double dmul = exp(-h.delta * h.delta);          // c_{j+1} / c_j
+    const double c0 = exp(-0.5 * h.delta * h.delta);      // c_0
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    visit_grid<SPLIT>(h, hpos, cstart, t, sub, xi, yi, zi,
+                      [&](double xj, double yj, double zj, double r2) {
+        const double r = sqrt(r2);
+        int k0, k1;
+        edge_window(h, r, k0, k1);
+        double sum = 0.0;
+        if (recur) {
+            const double t0 = (h.r_lo + h.width * k0 - r) * h.inv_sigma;
+            double phi = norm_pdf(t0);
+            const double A = exp(-h.delta * t0);
+            double cj = c0;
+            for (int k = k0; k <= k1; ++k) {
+                sum = fma(phi, dw[k], sum);
+                phi *= A * cj;
+                cj *= dmul;
+            }
+        } else {
+            for (int k = k0; k <= k1; ++k)
+                sum = fma(norm_pdf((h.r_lo + h.width * k - r) * h.inv_sigma), dw[k], sum);
+        }
+        const double S = sum * h.inv_sigma;
+        double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+        dx = dx > h.half ? dx - h.L : (dx < -h.half ? dx + h.L : dx);
+        dy = dy > h.half ? dy - h.L : (dy < -h.half ? dy + h.L : dy);
+        dz = dz > h.half ? dz - h.L : (dz < -h.half ? dz + h.L : dz);
+        const double c = -2.0 * S / r;
+        gx = fma(c, dx, gx);
+        gy = fma(c, dy, gy);
+        gz = fma(c, dz, gz);
+    });
+#pragma unroll
+    for (int d = 1; d < SPLIT; d <<= 1) {
+        gx += __shfl_xor(gx, d, 64);
+        gy += __shfl_xor(gy, d, 64);
+        gz += __shfl_xor(gz, d, 64);
+    }
+    if (sub == 0) {
+        const int o = id[perm[t]];
+        grad[o] = gx;
+        grad[N + o] = gy;
+        grad[2 * N + o] = gz;
+    }
 }
 
 // ---- host helpers -------------------------------------------------------------
@@ -374,6 +578,7 @@ int build_hist_grid(psim_ctx *c, double r_lo, double width, int nbins, double si
     h.width = width;
     h.sigma = sigma;
     h.inv_sigma = sigma > 0.0 ? 1.0 / sigma : 0.0;
+    h.delta = width * h.inv_sigma;
     h.reach2 = reach * reach;
     h.nbins = nbins;
     int S = (int)std::floor(L / reach);
@@ -389,7 +594,7 @@ int build_hist_grid(psim_ctx *c, double r_lo, double width, int nbins, double si
     auto &g = c->hg;
     const int ncell = S * S * S;
     if ((size_t)ncell > g.cap_cells || !g.key) {
-        void *bufs[] = {g.key, g.rank, g.perm, g.count, g.cstart, g.bsum};
+        void *bufs[] = {g.key, g.rank, g.perm, g.count, g.cstart, g.bsum, g.hpos};
         for (void *b : bufs)
             if (b) CK(hipFree(b));
         g = psim_ctx::HistGrid{};
@@ -401,6 +606,7 @@ int build_hist_grid(psim_ctx *c, double r_lo, double width, int nbins, double si
         if (!rc) rc = dalloc(&g.count, ncell);
         if (!rc) rc = dalloc(&g.cstart, (size_t)ncell + 1);
         if (!rc) rc = dalloc(&g.bsum, nscan);
+        if (!rc) rc = dalloc(&g.hpos, 3 * (size_t)n);
         if (rc) return rc;
         CK(hipMemsetAsync(g.count, 0, sizeof(int) * ncell, c->stream));
         g.cap_cells = ncell;
@@ -420,7 +626,8 @@ int build_hist_grid(psim_ctx *c, double r_lo, double width, int nbins, double si
     launch_scan3(st, g.count, ncell, g.nscan, g.bsum, g.cstart, n);
     k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, g.key, g.rank, g.cstart, g.perm + n,
                                                   nullptr, 0);
-    k_hg_canon<<<blocks_for(n, 256), 256, 0, st>>>(n, g.perm + n, g.key, g.cstart, g.perm);
+    k_hg_canon<<<blocks_for(n, 256), 256, 0, st>>>(n, g.perm + n, g.key, g.cstart, g.perm,
+                                                   c->pos, g.hpos);
     CK(hipGetLastError());
     return 0;
 }
@@ -574,10 +781,27 @@ int psim_gpu_pair_histogram(psim_ctx *c, double r_lo, double width, int nbins, d
     HistGeo h{};
     rc = build_hist_grid(c, r_lo, width, nbins, sigma, h);
     if (rc) return rc;
+    const int n = c->n;
+    if (h.S && sigma > 0.0) {
+        // per-block partials, summed in block order (reproducible)
+        const unsigned nblk = blocks_for((long long)n * kHistSplit, kAnaBlock);
+        rc = ensure_work(c, (size_t)nblk * nbins + nbins, 1);
+        if (rc) return rc;
+        double *part = c->work, *out = c->work + (size_t)nblk * nbins;
+        const size_t lds = sizeof(double) * (kHistCap * kAnaBlock + (kAnaBlock / 64) * nbins);
+        k_pair_hist_grid<kHistSplit><<<nblk, kAnaBlock, lds, c->stream>>>(h, c->hg.hpos,
+                                                                        c->hg.cstart, part);
+        k_hist_sum<<<nbins, 256, 0, c->stream>>>(part, (int)nblk, nbins, out);
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(hist, out, sizeof(double) * nbins, hipMemcpyDeviceToHost,
+                          c->stream));
+        CK(hipStreamSynchronize(c->stream));
+        return 0;
+    }
     rc = ensure_work(c, (size_t)nbins, 1);
     if (rc) return rc;
     CK(hipMemsetAsync(c->work, 0, sizeof(double) * nbins, c->stream));
-    k_pair_hist<<<blocks_for(c->n, kAnaBlock), kAnaBlock, 0, c->stream>>>(
+    k_pair_hist<<<blocks_for(n, kAnaBlock), kAnaBlock, 0, c->stream>>>(
         h, c->pos, c->hg.perm, c->hg.cstart, c->work);
     CK(hipGetLastError());
     CK(hipMemcpyAsync(hist, c->work, sizeof(double) * nbins, hipMemcpyDeviceToHost,
@@ -600,8 +824,15 @@ int psim_gpu_pair_histogram_grad(psim_ctx *c, double r_lo, double width, int nbi
     if (rc) return rc;
     double *dw = c->work, *dg = c->work + nbins;
     CK(hipMemcpyAsync(dw, w, sizeof(double) * nbins, hipMemcpyHostToDevice, c->stream));
-    k_pair_hist_grad<<<blocks_for(n, kAnaBlock), kAnaBlock, 0, c->stream>>>(
-        h, c->pos, c->hg.perm, c->hg.cstart, c->id, dw, dg);
+    if (h.S) {
+        const size_t lds = sizeof(double) * (nbins + 1);
+        k_pair_hist_grad_grid<kHistSplit>
+            <<<blocks_for((long long)n * kHistSplit, kAnaBlock), kAnaBlock, lds, c->stream>>>(
+                h, c->hg.hpos, c->hg.perm, c->hg.cstart, c->id, dw, dg);
+    } else {
+        k_pair_hist_grad<<<blocks_for(n, kAnaBlock), kAnaBlock, 0, c->stream>>>(
+            h, c->pos, c->hg.perm, c->hg.cstart, c->id, dw, dg);
+    }
     CK(hipGetLastError());
     std::vector<double> hg(3 * (size_t)n);
     CK(hipMemcpyAsync(hg.data(), dg, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,

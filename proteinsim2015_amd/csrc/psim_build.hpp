@@ -114,11 +114,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_bsum(int *bsum, int nb)
 }
 
 // Cell start table from the counts: block b scans its tile on top of the
-// prefix bsum[b] (k_scan_reduce, then k_scan_bsum), writes the starts and
-// clears the counts for the next step's counting.
+// prefix bsum[b] (k_scan_reduce, then k_scan_bsum), writes the starts (also
+// to cstart2 when set) and clears the counts for the next step's counting.
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(int *count, int ncell,
                                                              const int *bsum,
-                                                             int *cstart, int n)
+                                                             int *cstart, int n,
+                                                             int *cstart2)
 {
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int c[kScanItems];
@@ -128,28 +129,33 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(int *count, int nce
     if (base + kScanItems <= ncell) {
         const int4 st = make_int4(pre, pre + c[0], pre + c[0] + c[1], pre + c[0] + c[1] + c[2]);
         *reinterpret_cast<int4 *>(cstart + base) = st;
+        if (cstart2) *reinterpret_cast<int4 *>(cstart2 + base) = st;
         *reinterpret_cast<int4 *>(count + base) = make_int4(0, 0, 0, 0);
     } else {
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) {
             if (base + k < ncell) {
                 cstart[base + k] = pre;
+                if (cstart2) cstart2[base + k] = pre;
                 count[base + k] = 0;
             }
             pre += c[k];
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cstart[ncell] = n;
+        if (cstart2) cstart2[ncell] = n;
+    }
 }
 
 // The three-launch scan of the cell counts into the start table (grids of
 // more than kScanDirectTiles tiles).
 inline void launch_scan3(hipStream_t st, int *count, int ncell, int nscan, int *bsum,
-                         int *cstart, int n)
+                         int *cstart, int n, int *cstart2 = nullptr)
 {
     k_scan_reduce<<<nscan, kScanThreads, 0, st>>>(count, ncell, bsum);
     k_scan_bsum<<<1, kScanThreads, 0, st>>>(bsum, nscan);
-    k_scan_apply<<<nscan, kScanThreads, 0, st>>>(count, ncell, bsum, cstart, n);
+    k_scan_apply<<<nscan, kScanThreads, 0, st>>>(count, ncell, bsum, cstart, n, cstart2);
 }
 
 // Cell start table in one launch for grids of at most kScanDirectTiles
@@ -160,7 +166,8 @@ inline void launch_scan3(hipStream_t st, int *count, int ncell, int nscan, int *
 // 9.8 vs 9.5 us at 33)
 constexpr int kScanDirectTiles = 32;
 __global__ __launch_bounds__(kScanThreads) void k_scan_direct(const int *count, int ncell,
-                                                              int *cstart, int n)
+                                                              int *cstart, int n,
+                                                              int *cstart2 = nullptr)
 {
     const int lim = blockIdx.x * kScanTile;   // a multiple of 4, at most ncell
     int before = 0;
@@ -191,10 +198,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_direct(const int *count, 
     int pre = block_exclusive_scan(v, &total) + before_total;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < ncell) cstart[base + k] = pre;
+        if (base + k < ncell) {
+            cstart[base + k] = pre;
+            if (cstart2) cstart2[base + k] = pre;
+        }
         pre += c[k];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cstart[ncell] = n;
+        if (cstart2) cstart2[ncell] = n;
+    }
 }
 
 // With clear_count set (after k_scan_direct), also zeroes the ncell counts
@@ -253,12 +266,16 @@ __global__ void k_move(int n, const int *__restrict__ perm, const int *__restric
 // step.
 // With ctl set (a step sequence whose control lives in device memory), the
 // kernel also ends the step: it advances the clock and the next step's
-// flags, which no kernel of this step reads after the integrator.
+// flags, which no kernel of this step reads after the integrator.  With
+// pos2 / id2 set, the new positions and ids are also written there (a
+// record of the state, e.g. the adjoint's recomputed trajectory), and with
+// inv2 the slot of each original index.
 __global__ void k_move_lagged(int n, const int *__restrict__ perm, const int *__restrict__ key,
                               const int *__restrict__ cstart, const double *__restrict__ npos,
                               const int *__restrict__ id_in, double *__restrict__ pos,
                               int *__restrict__ id_out, int *__restrict__ prev,
-                              StepCtl *ctl)
+                              StepCtl *ctl, double *__restrict__ pos2 = nullptr,
+                              int *__restrict__ id2 = nullptr, int *__restrict__ inv2 = nullptr)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (ctl && t == 0) ctl_advance(ctl);
@@ -275,8 +292,15 @@ __global__ void k_move_lagged(int n, const int *__restrict__ perm, const int *__
     for (; u < e; ++u) rank += (perm[u] < s);
     const int dst = b + rank;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) pos[d * n + dst] = npos[d * n + s];
-    id_out[dst] = id_in[s];
+    for (int d = 0; d < 3; ++d) {
+        const double x = npos[d * n + s];
+        pos[d * n + dst] = x;
+        if (pos2) pos2[d * n + dst] = x;
+    }
+    const int o = id_in[s];
+    id_out[dst] = o;
+    if (id2) id2[dst] = o;
+    if (inv2) inv2[o] = dst;
     prev[dst] = s;
 }
 

@@ -79,8 +79,16 @@ struct psim_ctx {
         double cell = 0.0;
         int *key = nullptr, *rank = nullptr, *perm = nullptr;
         int *count = nullptr, *cstart = nullptr, *bsum = nullptr;
+        double *hpos = nullptr;   // positions in grid order, [3][n]
         size_t cap_cells = 0;
     } hg;
+    // when set, the cell rebuild also writes the new positions, ids and cell
+    // table there (the adjoint's record of a recomputed step)
+    struct Record {
+        double *pos = nullptr;
+        int *id = nullptr, *cstart = nullptr;
+        int *inv = nullptr;   // original index -> slot
+    } rec;
     double *work = nullptr;     // analysis scratch (partials, histograms)
     size_t work_cap = 0;        // doubles
     int *iwork = nullptr;       // analysis scratch (labels, flags)

@@ -92,9 +92,10 @@ int launch_build(psim_ctx *c, StepCtl *advance)
     hipStream_t st = c->stream;
     const bool direct = c->nscan <= kScanDirectTiles;
     if (direct) {
-        k_scan_direct<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->cstart, n);
+        k_scan_direct<<<c->nscan, kScanThreads, 0, st>>>(c->count, nc, c->cstart, n,
+                                                         c->rec.cstart);
     } else {
-        launch_scan3(st, c->count, nc, c->nscan, c->bsum, c->cstart, n);
+        launch_scan3(st, c->count, nc, c->nscan, c->bsum, c->cstart, n, c->rec.cstart);
     }
     k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, c->key, c->arank, c->cstart, c->perm,
                                                   direct ? c->count : nullptr, nc);
@@ -103,7 +104,8 @@ int launch_build(psim_ctx *c, StepCtl *advance)
     // nmem, old order) and the next step reads them through prev
     k_move_lagged<<<blocks_for(n, 256), 256, 0, st>>>(n, c->perm, c->key, c->cstart, c->npos,
                                                       c->id, c->pos, c->nid, c->prev,
-                                                      advance);
+                                                      advance, c->rec.pos, c->rec.id,
+                                                      c->rec.inv);
     CK(hipGetLastError());
     std::swap(c->id, c->nid);
     std::swap(c->pos0, c->npos0);
@@ -645,6 +647,7 @@ void psim_gpu_destroy(psim_ctx *c)
                     c->tsorted, c->thi, c->tcse, c->st_count, c->st_hash, c->pbc,
                     c->track0, c->hg.key,
                     c->hg.rank, c->hg.perm, c->hg.count, c->hg.cstart, c->hg.bsum,
+                    c->hg.hpos,
                     c->work, c->iwork};
     for (void *b : bufs)
         if (b) (void)hipFree(b);

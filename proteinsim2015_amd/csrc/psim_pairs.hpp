@@ -22,6 +22,12 @@
 
 namespace psim {
 
+// x-runs whose slot ranges are read together (1: one run at a time)
+#ifndef PSIM_RUN_GROUP
+#define PSIM_RUN_GROUP 3
+#endif
+constexpr int kRunGroup = PSIM_RUN_GROUP;
+
 // Positions in the engine's slot-ordered SoA ([3][N]) with one radius.
 struct SoAPos {
     const double *__restrict__ p;
@@ -104,8 +110,12 @@ __device__ __forceinline__ XRun x_run(const Tab &tab, int row, int cx, int S)
     XRun r;
     tab.row_range(row + (lw ? S - 1 : cx - 1), row + (lw || rw ? S - 1 : cx + 1), r.b0, r.e0);
     r.k0 = lw ? 1 : 0;
-    tab.row_range(row, row + (lw ? 1 : 0), r.b1, r.e1);
-    if (!(lw || rw)) r.e1 = r.b1;
+    if (lw || rw) {
+        tab.row_range(row, row + (lw ? 1 : 0), r.b1, r.e1);
+    } else {
+        r.b1 = 0;   // an interior row is one segment (no table reads for the second)
+        r.e1 = 0;
+    }
     r.k1 = lw ? 0 : 2;
     return r;
 }
@@ -233,20 +243,38 @@ __device__ void gather_force(const Geo &g, const Pot &p, const Src &src,
     const int cz = cell_1d(zi, g.cell, S);
     int nh = 0;
     // SPLIT > 1 (few particles): lane `sub` takes runs sub, sub + SPLIT, ...
-    for (int q = sub; q < 9; q += SPLIT) {
-        int ry = cy + q / 3 - 1, rz = cz + q % 3 - 1;
-        int ky = 0, kz = 0;
-        if (ry < 0) { ry += S; ky = 1; } else if (ry >= S) { ry -= S; ky = 2; }
-        if (rz < 0) { rz += S; kz = 1; } else if (rz >= S) { rz -= S; kz = 2; }
-        const double ys = shifted(yi, ky, g.L), zs = shifted(zi, kz, g.L);
-        const XRun xr = x_run(tab, g.c0 + S * ry + g.S2 * rz, cx, S);
+    // The slot ranges of up to three runs are read before the first of them
+    // is culled (their table reads are in flight together).
+    for (int q0 = sub; q0 < 9; q0 += kRunGroup * SPLIT) {
+        XRun xr[kRunGroup];
+        int kyz[kRunGroup];
 #pragma unroll
-        for (int sg = 0; sg < 2; ++sg) {
-            const int kx = sg ? xr.k1 : xr.k0;
-            cull_cell<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, shifted(xi, kx, g.L), ys, zs,
-                                          sg ? xr.b1 : xr.b0, sg ? xr.e1 : xr.e0,
-                                          (unsigned)(kx + 3 * ky + 9 * kz) << kIdxBits, hits,
-                                          nh, ri, fx, fy, fz, err, orig, st);
+        for (int u = 0; u < kRunGroup; ++u) {
+            const int q = q0 + u * SPLIT;
+            xr[u] = XRun{0, 0, 0, 0, 0, 0};
+            kyz[u] = 0;
+            if (q < 9) {
+                int ry = cy + q / 3 - 1, rz = cz + q % 3 - 1;
+                int ky = 0, kz = 0;
+                if (ry < 0) { ry += S; ky = 1; } else if (ry >= S) { ry -= S; ky = 2; }
+                if (rz < 0) { rz += S; kz = 1; } else if (rz >= S) { rz -= S; kz = 2; }
+                xr[u] = x_run(tab, g.c0 + S * ry + g.S2 * rz, cx, S);
+                kyz[u] = ky + 3 * kz;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRunGroup; ++u) {
+            const int ky = kyz[u] % 3, kz = kyz[u] / 3;
+            const double ys = shifted(yi, ky, g.L), zs = shifted(zi, kz, g.L);
+#pragma unroll
+            for (int sg = 0; sg < 2; ++sg) {
+                const int kx = sg ? xr[u].k1 : xr[u].k0;
+                cull_cell<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, shifted(xi, kx, g.L), ys,
+                                              zs, sg ? xr[u].b1 : xr[u].b0,
+                                              sg ? xr[u].e1 : xr[u].e0,
+                                              (unsigned)(kx + 3 * ky + 9 * kz) << kIdxBits,
+                                              hits, nh, ri, fx, fy, fz, err, orig, st);
+            }
         }
     }
     drain_hits<MASK, EXACT, BLOCK>(g, p, src, s, xi, yi, zi, ri, hits, nh, fx, fy, fz, err,
