@@ -114,18 +114,29 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_bsum(int *bsum, int nb)
 }
 
 // Cell start table from the counts: block b scans its tile on top of the
-// prefix bsum[b] (k_scan_reduce, then k_scan_bsum), writes the starts (also
-// to cstart2 when set) and clears the counts for the next step's counting.
+// prefix of the tile totals before it, writes the starts (also to cstart2
+// when set) and clears the counts for the next step's counting.  With
+// RESUM, the block sums the raw totals bsum[0..b) itself (few tiles);
+// otherwise bsum already holds the prefixes (k_scan_bsum).
+template <bool RESUM>
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(int *count, int ncell,
                                                              const int *bsum,
                                                              int *cstart, int n,
                                                              int *cstart2)
 {
+    int before;
+    if constexpr (RESUM) {
+        int v = 0;
+        for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) v += bsum[b];
+        block_exclusive_scan(v, &before);
+    } else {
+        before = bsum[blockIdx.x];
+    }
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int c[kScanItems];
     tile_counts(count, ncell, base, c);
     int total;
-    int pre = block_exclusive_scan(c[0] + c[1] + c[2] + c[3], &total) + bsum[blockIdx.x];
+    int pre = block_exclusive_scan(c[0] + c[1] + c[2] + c[3], &total) + before;
     if (base + kScanItems <= ncell) {
         const int4 st = make_int4(pre, pre + c[0], pre + c[0] + c[1], pre + c[0] + c[1] + c[2]);
         *reinterpret_cast<int4 *>(cstart + base) = st;
@@ -148,14 +159,24 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(int *count, int nce
     }
 }
 
-// The three-launch scan of the cell counts into the start table (grids of
-// more than kScanDirectTiles tiles).
+// The scan of the cell counts into the start table for grids of more than
+// kScanDirectTiles tiles: tile totals, then each tile's scan on top of the
+// totals before it, summed by each block itself up to kScanResumTiles tiles
+// (two launches), beyond that first turned into prefixes by one block
+// (three launches; the re-summing grows with the square of the tile count).
+constexpr int kScanResumTiles = 256;
 inline void launch_scan3(hipStream_t st, int *count, int ncell, int nscan, int *bsum,
                          int *cstart, int n, int *cstart2 = nullptr)
 {
     k_scan_reduce<<<nscan, kScanThreads, 0, st>>>(count, ncell, bsum);
-    k_scan_bsum<<<1, kScanThreads, 0, st>>>(bsum, nscan);
-    k_scan_apply<<<nscan, kScanThreads, 0, st>>>(count, ncell, bsum, cstart, n, cstart2);
+    if (nscan <= kScanResumTiles) {
+        k_scan_apply<true><<<nscan, kScanThreads, 0, st>>>(count, ncell, bsum, cstart, n,
+                                                           cstart2);
+    } else {
+        k_scan_bsum<<<1, kScanThreads, 0, st>>>(bsum, nscan);
+        k_scan_apply<false><<<nscan, kScanThreads, 0, st>>>(count, ncell, bsum, cstart, n,
+                                                            cstart2);
+    }
 }
 
 // Cell start table in one launch for grids of at most kScanDirectTiles

@@ -81,6 +81,8 @@ __global__ __launch_bounds__(BLOCK) void k_ens_step(const RepGeo *__restrict__ r
     const double radius = reps[r].radius;
     const int N = ntot;
     const double xi = a.pos[s], yi = a.pos[N + s], zi = a.pos[2 * N + s];
+    // the integrator's index loads, in flight during the force loop
+    const int o = a.id[s], ps = a.prev ? a.prev[s] : s;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     SoAPos src{a.pos, N, radius};
     StartTab tab{a.cstart};
@@ -91,7 +93,7 @@ __global__ __launch_bounds__(BLOCK) void k_ens_step(const RepGeo *__restrict__ r
     else
         gather_force<MASK, false, BLOCK>(g, p, src, tab, s, xi, yi, zi, radius,
                                          hits + threadIdx.x, fx, fy, fz, a.err, orig);
-    integrate_one<true>(g, gb, a, s, fx, fy, fz);
+    integrate_one<true>(g, gb, a, s, fx, fy, fz, o, ps);
 }
 
 // Brick kernel: workgroup `it` of the launch is brick it - item0[r] of

@@ -134,7 +134,9 @@ __device__ inline double shifted(double xi, int k, double L)
 }
 
 // Drain one lane's hit list: pair forces for entries [0, nh).  The list is a
-// column of stride BLOCK in the block's LDS array.
+// column of stride BLOCK in the block's LDS array.  Entries are taken two at
+// a time: both partners' positions are loaded before either pair is
+// evaluated (the evaluation order stays the list order).
 template <int MASK, bool EXACT, int BLOCK = kBlock, class Src, class ST = NoStats>
 __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
                            double xi, double yi, double zi, double ri,
@@ -142,11 +144,9 @@ __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
                            double &fy, double &fz, DevErr *err,
                            const Orig &orig, ST *st = nullptr)
 {
-    for (int k = 0; k < nh; ++k) {
-        unsigned e = hits[k * BLOCK];
-        int j = (int)(e & kIdxMask);
-        double xj = src.x(j), yj = src.y(j), zj = src.z(j);
-        double size = ri + src.rad(j);
+    auto pair = [&](unsigned e, double xj, double yj, double zj) {
+        const int j = (int)(e & kIdxMask);
+        const double size = ri + src.rad(j);
         double dx, dy, dz, r2, rinv, r;
         if (!EXACT) {
             unsigned code = e >> kIdxBits;
@@ -157,7 +157,7 @@ __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
             r2 = fma(dx, dx, fma(dy, dy, dz * dz));
             if (r2 >= p.rc2_lo) {
                 double r2x = pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc);
-                if (!(r2x < p.rc2)) continue;
+                if (!(r2x < p.rc2)) return;
             }
             rinv = rsqrt_nr(r2);
             r = r2 * rinv;
@@ -172,7 +172,7 @@ __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
         }
         if constexpr (ST::kOn) {
             st->add(orig(j));
-            continue;
+            return;
         }
         double ov = p.ov_factor * size;
         if (r2 < ov * ov * (1.0 + p.ov_band)) {
@@ -185,6 +185,27 @@ __device__ void drain_hits(const Geo &g, const Pot &p, const Src &src, int s,
         fx = fma(fr, dx, fx);
         fy = fma(fr, dy, fy);
         fz = fma(fr, dz, fz);
+    };
+    // (one at a time with LJ and AO both enabled: the pair arithmetic then
+    // leaves no registers for a second partner)
+    constexpr bool kTwo = !((MASK & F_LJ) && (MASK & F_AO));
+    if constexpr (kTwo) {
+        for (int k = 0; k < nh; k += 2) {
+            const bool two = k + 1 < nh;
+            const unsigned e0 = hits[k * BLOCK];
+            const unsigned e1 = two ? hits[(k + 1) * BLOCK] : e0;
+            const int j0 = (int)(e0 & kIdxMask), j1 = (int)(e1 & kIdxMask);
+            const double x0 = src.x(j0), y0 = src.y(j0), z0 = src.z(j0);
+            const double x1 = src.x(j1), y1 = src.y(j1), z1 = src.z(j1);
+            pair(e0, x0, y0, z0);
+            if (two) pair(e1, x1, y1, z1);
+        }
+    } else {
+        for (int k = 0; k < nh; ++k) {
+            const unsigned e = hits[k * BLOCK];
+            const int j = (int)(e & kIdxMask);
+            pair(e, src.x(j), src.y(j), src.z(j));
+        }
     }
 }
 

@@ -87,12 +87,14 @@ struct StepArgs {
 // G&B epilogue for the particle in slot s with total force F
 // (brownianIntegrator.cpp:191-356, particle.cpp:61-99); writes the next state
 // at slot s, the new cell key and an atomic per-cell rank.
+// (o = id[s] and ps = prev[s] may be read by the caller ahead of the force
+// loop, so that their latency overlaps it: pass them as o_pre / ps_pre.)
 template <bool WRITE_F>
 __device__ __forceinline__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int s,
-                              double fx, double fy, double fz)
+                              double fx, double fy, double fz, int o_pre = -1, int ps_pre = -1)
 {
     const int N = g.n;
-    const int o = a.id[s];
+    const int o = o_pre >= 0 ? o_pre : a.id[s];
     const int first = a.ctl ? a.ctl->first : a.first;
     const int do_vel = a.ctl ? a.ctl->do_vel : a.do_vel;
     double z[6];
@@ -105,7 +107,7 @@ __device__ __forceinline__ void integrate_one(const Geo &g, const GB &gb, const 
     const double xc[3] = {a.pos[s], a.pos[N + s], a.pos[2 * N + s]};
     const double F[3] = {fx, fy, fz};
     // the previous position, force and memory where the last step left them
-    const int ps = a.prev ? a.prev[s] : s;
+    const int ps = ps_pre >= 0 ? ps_pre : (a.prev ? a.prev[s] : s);
     double x0c[3], xn[3], M[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) x0c[d] = a.pos0[d * N + ps];
@@ -198,6 +200,8 @@ __global__ __launch_bounds__(BLOCK) void k_step(Geo g, Pot p, GB gb, double radi
     if (s >= g.n) return;   // whole lane groups
     const int N = g.n;
     const double xi = a.pos[s], yi = a.pos[N + s], zi = a.pos[2 * N + s];
+    // the integrator's index loads, in flight during the force loop
+    const int o = a.id[s], ps = a.prev ? a.prev[s] : s;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     SoAPos src{a.pos, N, radius};
     StartTab tab{a.cstart};
@@ -216,7 +220,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(Geo g, Pot p, GB gb, double radi
         fz = split_sum<SPLIT>(fz);
         if (sub != 0) return;
     }
-    integrate_one<true>(g, gb, a, s, fx, fy, fz);
+    integrate_one<true>(g, gb, a, s, fx, fy, fz, o, ps);
 }
 
 // Brick-per-workgroup step (sparse systems): G lanes per home particle, the
