@@ -371,11 +371,13 @@ int psim_gpu_pair_histogram_grad(psim_ctx *ctx, double r_lo, double width, int n
  * Philox noise, so the noise is fixed by the seed and step index) with
  * respect to the pair-force parameters.  No reference counterpart: the
  * reference has no derivatives; the forward steps are psim_gpu_step's.
- * The derivative is pathwise: the set of pairs inside the cutoff is the one
- * the forward trajectory used.  dtheta[PSIM_DP_*] receives dL/dparameter. */
+ * By default the derivative is pathwise: the set of pairs inside the cutoff
+ * is the one the forward trajectory used.  dtheta[PSIM_DP_*] receives
+ * dL/dparameter. */
 enum {
     PSIM_DP_AO_WELL = 0,      /* AO wellDepth (polymer fraction phi_p)       */
-    PSIM_DP_AO_CUTOFF = 1,    /* cutOff through the AO coefficients          */
+    PSIM_DP_AO_CUTOFF = 1,    /* cutOff through the AO coefficients (and, with
+                                 a cutoff window, through the pairs that cross) */
     PSIM_DP_LJ_WELL = 2,      /* LJ wellDepth (epsilon)                      */
     PSIM_DP_YUKAWA = 3,       /* yukawaStrength                              */
     PSIM_DP_DEBYE = 4,        /* debyeLength                                 */
@@ -388,6 +390,16 @@ typedef struct psim_adjoint psim_adjoint;
  * (nsteps/checkpoint_every + 1) * 16 N doubles + checkpoint_every * 4 N). */
 int psim_adj_create(psim_ctx *ctx, int nsteps, int checkpoint_every, psim_adjoint **out);
 void psim_adj_destroy(psim_adjoint *adj);
+/* Cutoff window w >= 0 (default 0, pathwise).  The pair force jumps at the
+ * cutoff (fNet(cutOff) != 0), so in a large system every change of cutOff
+ * moves pairs across it and the loss's secant slope holds those jumps.  With
+ * w > 0, dtheta[PSIM_DP_AO_CUTOFF] also receives the boundary term
+ * sum over steps and ordered pairs with |r - cutOff| < w of
+ * lambda_i . fNet(r) d_ij / r / (2 w): the derivative linearised over
+ * [cutOff - w, cutOff + w], comparable with a central difference of that
+ * half-width.  cutOff + w must not exceed the cell edge (else backward
+ * returns PSIM_E_ARG). */
+int psim_adj_set_cutoff_window(psim_adjoint *adj, double w);
 /* Re-take the start state from the context's current state. */
 int psim_adj_set_initial(psim_adjoint *adj);
 /* Run the trajectory from the start state with the context's current

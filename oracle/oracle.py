@@ -120,7 +120,7 @@ class Oracle:
                                  C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
                                  C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp,
                                  C.c_int, C.c_ulonglong, C.c_ulonglong, _dp, _dp,
-                                 C.c_void_p, C.c_void_p, C.c_int]
+                                 C.c_void_p, C.c_void_p, C.c_int, C.c_double]
         L.po_adjoint.restype = C.c_int
 
     # -- geometry / cells -------------------------------------------------
@@ -326,8 +326,9 @@ class Oracle:
         return f.value, d.value, th
 
     def adjoint(self, fp, c, state, nsteps, box, S, cell_size, gx, radius=0.5,
-                vel_freq=1000, seed=0, half_uv=None, nthreads=0):
-        """dL/dtheta (5), dL/dx_0 and x_K for nsteps from ``state`` (not modified)."""
+                vel_freq=1000, seed=0, half_uv=None, nthreads=0, cut_window=0.0):
+        """dL/dtheta (5), dL/dx_0 and x_K for nsteps from ``state`` (not
+        modified); cut_window > 0 adds the cutoff-crossing term to dL/dcutOff."""
         n = state["pos"].shape[0]
         if half_uv is None:
             half_uv = float(int(box) // 2) if float(box).is_integer() else box / 2.0
@@ -343,7 +344,8 @@ class Oracle:
                                  arr["vel"], arr["frc"], arr["memx"], int(nsteps), int(seed),
                                  int(state.get("step_index", 0)), _f64(gx).reshape(-1), dth,
                                  gx0.ctypes.data_as(C.c_void_p),
-                                 xk.ctypes.data_as(C.c_void_p), int(nthreads))
+                                 xk.ctypes.data_as(C.c_void_p), int(nthreads),
+                                 float(cut_window))
         return st, dth, gx0.reshape(-1, 3), xk.reshape(-1, 3)
 
 

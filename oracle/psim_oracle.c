@@ -1001,14 +1001,16 @@ void po_pair_fnet_d(const po_force_params *fp, double r, double size, double *f,
  * every step's positions (test sizes).  pos_out (may be NULL) receives x_K.
  * Recurrence: lambda_n = alpha_n a_{n+1} - c2 a_{n+2},
  * a_n = beta_n a_{n+1} - coeff0 a_{n+2} + J(x_n) lambda_n,
- * dL/dtheta = sum_n lambda_n . dF(x_n)/dtheta. */
+ * dL/dtheta = sum_n lambda_n . dF(x_n)/dtheta.  With cut_window w > 0 the
+ * cutOff derivative also holds the pairs crossing the cutoff:
+ * dF_i/dc += sum_j 1{|r_ij - c| < w} / (2 w) fNet(r_ij) d_ij / r_ij. */
 int po_adjoint(const po_force_params *fp, const po_gb_coeffs *c, int n, double box,
                double half_uv, int S, double cell_size, double radius, int vel_freq,
                int vel_counter, double current_time, const double *pos_in,
                const double *pos0_in, const double *vel_in, const double *frc_in,
                const double *memx_in, int nsteps, unsigned long long seed,
                unsigned long long step_index, const double *gx, double *dtheta, double *gx0,
-               double *pos_out, int nthreads)
+               double *pos_out, int nthreads, double cut_window)
 {
     size_t n3 = 3 * (size_t)n;
     int ncell = S * S * S;
@@ -1074,7 +1076,9 @@ int po_adjoint(const po_force_params *fp, const po_gb_coeffs *c, int n, double b
                             if (i == index) continue;
                             const double *b = sorted + 4 * i;
                             double r2 = po_pbc_dist(a[0], a[1], a[2], b[0], b[1], b[2], box);
-                            if (!(r2 < fp->cutoff * fp->cutoff)) continue;
+                            double cw = fp->cutoff + cut_window;
+                            int inside = r2 < fp->cutoff * fp->cutoff;
+                            if (!inside && !(cut_window > 0.0 && r2 < cw * cw)) continue;
                             double r = sqrt(r2);
                             double u[3];
                             po_unit_vector(a[0], a[1], a[2], b[0], b[1], b[2], u, r, box,
@@ -1090,6 +1094,9 @@ int po_adjoint(const po_force_params *fp, const po_gb_coeffs *c, int n, double b
                                 dd += d[k] * e[k];
                                 ld += lam[3 * oi + k] * d[k];
                             }
+                            if (fabs(r - fp->cutoff) < cut_window)
+                                th[1] += f / (2.0 * cut_window) / r * ld;
+                            if (!inside) continue;
                             for (int k = 0; k < 3; k++)
                                 jl[3 * oi + k] += g * e[k] + gp / r * dd * d[k];
                             for (int q = 0; q < 5; q++) th[q] += dth[q] / r * ld;

@@ -199,14 +199,14 @@ void po_pair_fnet_d(const po_force_params *fp, double r, double size, double *f,
                     double *dfdr, double dth[5]);
 /* Trajectory adjoint: nsteps of po_run with Philox noise, loss gradient gx
  * at the last step -> dtheta[5], dL/dx_0 (gx0, may be NULL), x_K (pos_out,
- * may be NULL). */
+ * may be NULL); cut_window > 0 adds the cutoff-crossing term to dtheta[1]. */
 int po_adjoint(const po_force_params *fp, const po_gb_coeffs *c, int n, double box,
                double half_uv, int S, double cell_size, double radius, int vel_freq,
                int vel_counter, double current_time, const double *pos_in,
                const double *pos0_in, const double *vel_in, const double *frc_in,
                const double *memx_in, int nsteps, unsigned long long seed,
                unsigned long long step_index, const double *gx, double *dtheta, double *gx0,
-               double *pos_out, int nthreads);
+               double *pos_out, int nthreads, double cut_window);
 
 #ifdef __cplusplus
 }

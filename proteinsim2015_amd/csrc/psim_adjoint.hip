@@ -15,10 +15,16 @@
 // J_n = dF/dx at x_n (symmetric): for a pair force F_i = sum_j g(r) d_ij
 // with d_ij = x_j - x_i, (J lambda)_i = sum_j g (lambda_j - lambda_i)
 //   + (g'/r) (d . (lambda_j - lambda_i)) d.
-// The derivative is pathwise: the set of pairs inside the cutoff is the one
-// the forward trajectory used (a change of the cutoff moves no pair across
-// it), which is the exact derivative of the discrete trajectory wherever no
-// pair sits exactly at the cutoff.
+// By default the derivative is pathwise: the set of pairs inside the cutoff
+// is the one the forward trajectory used (a change of the cutoff moves no
+// pair across it), which is the exact derivative of the discrete trajectory
+// wherever no pair sits exactly at the cutoff.  The force jumps at the cutoff
+// (fNet(c) != 0), so the loss of a large system is a staircase in c whose
+// secant slope holds the pairs that cross.  With a cutoff window w > 0 the
+// cutoff derivative also carries that boundary term,
+//   dF_i/dc += sum_j delta(r_ij - c) fNet(r_ij) d_ij / r_ij,
+// with delta replaced by the box 1{|r - c| < w} / (2 w): the linearised
+// secant over [c - w, c + w].
 //
 // Memory: a checkpoint of the full state every `every` steps; the backward
 // pass recomputes each segment from its checkpoint (the in-kernel Philox
@@ -49,11 +55,18 @@ struct AdjPot {
     double neg_kt;                      // -kT
     double ao_dc1_dphi, ao_dc2_dphi;    // dC1/dwellDepth, dC2/dwellDepth
     double ao_dc1_dc, ao_dc2_dc;        // dC1/dcutOff, dC2/dcutOff
+    // cutoff window: pairs with |r - c| < win add fNet / (2 win) to the
+    // cutoff derivative; win_hi2 = (c + win)^2 (0 without a window)
+    double cut, win, win_hi2, win_scale;
 };
 
-AdjPot make_adj(const psim_params &P)
+AdjPot make_adj(const psim_params &P, double win)
 {
     AdjPot q{};
+    q.cut = P.cutoff;
+    q.win = win;
+    q.win_hi2 = win > 0.0 ? (P.cutoff + win) * (P.cutoff + win) : 0.0;
+    q.win_scale = win > 0.0 ? 0.5 / win : 0.0;
     q.neg_kt = -P.kT;
     const double rc = P.cutoff;
     const double K = -P.kT * std::pow(rc / (rc - 1.0), 3);
@@ -115,12 +128,14 @@ __device__ __forceinline__ void pair_fnet_d(const Pot &p, const AdjPot &q, doubl
 }
 
 // Pairs of slot s inside the cutoff, with the forward kernels' neighbour
-// test and displacement (psim_pairs.hpp gather_force / drain_hits).  Lane
-// `sub` of the SPLIT lanes that share the particle visits the x-runs sub,
-// sub + SPLIT, ... of the 9 (gather_force's walk).
+// test and displacement (psim_pairs.hpp gather_force / drain_hits), as
+// fn(j, dx, dy, dz, r, 1/r, true); with win_hi2 > 0 also the pairs outside
+// it with r^2 < win_hi2, as fn(..., false).  Lane `sub` of the SPLIT lanes
+// that share the particle visits the x-runs sub, sub + SPLIT, ... of the 9
+// (gather_force's walk).
 template <bool EXACT, int SPLIT, class Fn>
 __device__ void visit_pairs(const Geo &g, const Pot &p, const double *pos, const int *cstart,
-                            int s, int sub, Fn fn)
+                            int s, int sub, double win_hi2, Fn fn)
 {
     const int N = g.n, S = g.S;
     const double xi = pos[s], yi = pos[N + s], zi = pos[2 * N + s];
@@ -142,27 +157,29 @@ __device__ void visit_pairs(const Geo &g, const Pot &p, const double *pos, const
                 if (j == s) continue;
                 const double xj = pos[j], yj = pos[N + j], zj = pos[2 * N + j];
                 double dx, dy, dz, r2, r, rinv;
+                bool inside = true;
                 if (!EXACT) {
                     dx = xj - xs;
                     dy = yj - ys;
                     dz = zj - zs;
                     r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                    if (!(r2 < p.rc2_hi)) continue;
-                    if (r2 >= p.rc2_lo &&
-                        !(pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc) < p.rc2))
-                        continue;
+                    if (!(r2 < p.rc2_hi) && !(r2 < win_hi2)) continue;
+                    if (r2 >= p.rc2_lo)
+                        inside = pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc) < p.rc2;
+                    if (!inside && !(r2 < win_hi2)) continue;
                     rinv = rsqrt_nr(r2);
                     r = r2 * rinv;
                 } else {
                     r2 = pbc_dist_ref(xi, yi, zi, xj, yj, zj, g.L, g.half_pbc);
-                    if (!(r2 < p.rc2)) continue;
+                    inside = r2 < p.rc2;
+                    if (!inside && !(r2 < win_hi2)) continue;
                     r = sqrt(r2);
                     rinv = 1.0 / r;
                     dx = min_image_uv(xj - xi, g.L, g.half_uv);
                     dy = min_image_uv(yj - yi, g.L, g.half_uv);
                     dz = min_image_uv(zj - zi, g.L, g.half_uv);
                 }
-                fn(j, dx, dy, dz, r, rinv);
+                fn(j, dx, dy, dz, r, rinv, inside);
             }
         }
     }
@@ -188,12 +205,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_step(
     if (s < N) {   // whole lane groups
         const double lx = lam[s], ly = lam[N + s], lz = lam[2 * N + s];
         const double size = 2.0 * radius;
-        visit_pairs<EXACT, SPLIT>(g, p, pos, cstart, s, sub,
+        visit_pairs<EXACT, SPLIT>(g, p, pos, cstart, s, sub, q.win_hi2,
                                   [&](int j, double dx, double dy, double dz, double r,
-                                      double rinv) {
-            const double ex = lam[j] - lx, ey = lam[N + j] - ly, ez = lam[2 * N + j] - lz;
+                                      double rinv, bool inside) {
             double f, fp, dth[kNTheta];
             pair_fnet_d<MASK>(p, q, r, rinv, size, f, fp, dth);
+            const double ld = (lx * dx + ly * dy + lz * dz) * rinv;
+            if (fabs(r - q.cut) < q.win) th[1] += f * q.win_scale * ld;
+            if (!inside) return;
+            const double ex = lam[j] - lx, ey = lam[N + j] - ly, ez = lam[2 * N + j] - lz;
             const double gr = f * rinv;
             const double gp = (fp - gr) * rinv;
             const double dd = dx * ex + dy * ey + dz * ez;
@@ -201,7 +221,6 @@ __global__ __launch_bounds__(BLOCK) void k_adj_step(
             jx += gr * ex + k2 * dx;
             jy += gr * ey + k2 * dy;
             jz += gr * ez + k2 * dz;
-            const double ld = (lx * dx + ly * dy + lz * dz) * rinv;
 #pragma unroll
             for (int k = 0; k < kNTheta; ++k) th[k] += dth[k] * ld;
         });
@@ -352,6 +371,7 @@ struct psim_adjoint {
     std::vector<int> first;        // per step: first-step flag
     bool have_forward = false;
     psim_params prm_used{};
+    double win = 0.0;   // cutoff window (0: pathwise)
 };
 
 namespace {
@@ -454,8 +474,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_stats(Geo g, Pot p, const double 
     const int s = gt / SPLIT, sub = gt % SPLIT;
     if (s >= g.n) return;   // whole lane groups
     NbrStats st;
-    visit_pairs<EXACT, SPLIT>(g, p, pos, cstart, s, sub,
-                              [&](int j, double, double, double, double, double) {
+    visit_pairs<EXACT, SPLIT>(g, p, pos, cstart, s, sub, 0.0,
+                              [&](int j, double, double, double, double, double, bool) {
                                   st.add(id[j]);
                               });
     split_stats<SPLIT>(st);
@@ -562,6 +582,13 @@ void psim_adj_destroy(psim_adjoint *A)
     delete A;
 }
 
+int psim_adj_set_cutoff_window(psim_adjoint *A, double w)
+{
+    if (!A || !(w >= 0.0) || !std::isfinite(w)) return PSIM_E_ARG;
+    A->win = w;
+    return 0;
+}
+
 int psim_adj_set_initial(psim_adjoint *A)
 {
     if (!A) return PSIM_E_ARG;
@@ -603,6 +630,8 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
     if (!A || !gx || !dtheta) return PSIM_E_ARG;
     psim_ctx *c = A->c;
     if (!A->have_forward || !same_forces(A->prm_used, c->prm)) return PSIM_E_ARG;
+    // the window's pairs must lie in the 27 cells the walk visits
+    if (A->win > 0.0 && c->prm.cutoff + A->win > c->geo.cell) return PSIM_E_ARG;
     TrackerPause pause(c);
     const int n = c->n;
     const size_t n3 = 3 * (size_t)n;
@@ -611,7 +640,7 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
     const double alpha_normal = gb.c1 + gb.c2;
     auto alpha = [&](int step) { return A->first[step] ? alpha_first : alpha_normal; };
     auto beta = [&](int step) { return A->first[step] ? 1.0 : gb.c0; };
-    const AdjPot q = make_adj(c->prm);
+    const AdjPot q = make_adj(c->prm, A->win);
     // a_K = gx, a_{K+1} = 0, lambda_{K-1} = alpha_{K-1} a_K
     {
         std::vector<double> h(n3);

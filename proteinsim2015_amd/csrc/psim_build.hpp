@@ -183,9 +183,11 @@ inline void launch_scan3(hipStream_t st, int *count, int ncell, int nscan, int *
 // tiles: each block sums the counts of the tiles before it itself instead of
 // reading k_scan_reduce's tile sums.  The counts stay in place (other blocks
 // are still reading them); k_scatter clears them.
-// (measured against k_scan_reduce + k_scan_apply: 7.0 vs 9.6 us at 20 tiles,
-// 9.8 vs 9.5 us at 33)
-constexpr int kScanDirectTiles = 32;
+// (alone it beats k_scan_reduce + k_scan_apply up to about 32 tiles, 7.0 vs
+// 9.6 us at 20; inside a step it does not: at 16 tiles, N = 32768 AO with
+// 64000 cells, the two launches make the whole step 3.5 us faster, so the
+// direct form is kept for the smallest grids only)
+constexpr int kScanDirectTiles = 4;
 __global__ __launch_bounds__(kScanThreads) void k_scan_direct(const int *count, int ncell,
                                                               int *cstart, int n,
                                                               int *cstart2 = nullptr)

@@ -425,6 +425,19 @@ __device__ inline double u01(uint32_t v)
     return ((double)v + 0.5) * 2.3283064365386963e-10;
 }
 
+// One Box-Muller pair q (0, 1, 2) of philox_normals6: normals 2q and 2q + 1.
+__device__ inline void philox_pair(unsigned long long seed, unsigned long long step, uint32_t id,
+                                   int q, double &z0, double &z1)
+{
+    uint32_t c[4] = {id, (uint32_t)step, (uint32_t)(step >> 32), q < 2 ? 0u : 1u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint32_t ur = (q == 1) ? c[2] : c[0], ua = (q == 1) ? c[3] : c[1];
+    const double rr = sqrt(-2.0 * log(u01(ur)));
+    const double th = 6.283185307179586 * u01(ua);
+    z0 = rr * cos(th);
+    z1 = rr * sin(th);
+}
+
 __device__ inline void philox_normals6(unsigned long long seed,
                                        unsigned long long step, uint32_t id,
                                        double z[6])

@@ -91,14 +91,18 @@ struct StepArgs {
 // loop, so that their latency overlaps it: pass them as o_pre / ps_pre.)
 template <bool WRITE_F>
 __device__ __forceinline__ void integrate_one(const Geo &g, const GB &gb, const StepArgs &a, int s,
-                              double fx, double fy, double fz, int o_pre = -1, int ps_pre = -1)
+                              double fx, double fy, double fz, int o_pre = -1, int ps_pre = -1,
+                              const double *z_pre = nullptr)
 {
     const int N = g.n;
     const int o = o_pre >= 0 ? o_pre : a.id[s];
     const int first = a.ctl ? a.ctl->first : a.first;
     const int do_vel = a.ctl ? a.ctl->do_vel : a.do_vel;
     double z[6];
-    if (a.noise) {
+    if (z_pre) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) z[k] = z_pre[k];
+    } else if (a.noise) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) z[k] = a.noise[(size_t)k * N + o];
     } else {
@@ -214,7 +218,30 @@ __global__ __launch_bounds__(BLOCK) void k_step(Geo g, Pot p, GB gb, double radi
         gather_force<MASK, false, BLOCK, SPLIT>(g, p, src, tab, s, xi, yi, zi, radius,
                                                 hits + threadIdx.x, fx, fy, fz, a.err, orig,
                                                 sub);
-    if constexpr (SPLIT > 1) {
+    // (not with LJ: its pair arithmetic leaves no registers for the pairs)
+    if constexpr (SPLIT >= 4 && !(MASK & F_LJ)) {
+        fx = split_sum<SPLIT>(fx);
+        fy = split_sum<SPLIT>(fy);
+        fz = split_sum<SPLIT>(fz);
+        if (!a.noise) {
+            // in-kernel noise: lanes 0..2 of the group each make one
+            // Box-Muller pair, gathered on lane 0
+            double za = 0.0, zb = 0.0;
+            if (sub < 3)
+                philox_pair(a.seed, a.ctl ? a.ctl->step : a.step, (uint32_t)o, sub, za, zb);
+            const int base = (int)(threadIdx.x & 63u) & ~(SPLIT - 1);
+            double z[6];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                z[2 * q] = __shfl(za, base + q, 64);
+                z[2 * q + 1] = __shfl(zb, base + q, 64);
+            }
+            if (sub != 0) return;
+            integrate_one<true>(g, gb, a, s, fx, fy, fz, o, ps, z);
+            return;
+        }
+        if (sub != 0) return;
+    } else if constexpr (SPLIT > 1) {
         fx = split_sum<SPLIT>(fx);
         fy = split_sum<SPLIT>(fy);
         fz = split_sum<SPLIT>(fz);

@@ -566,6 +566,12 @@ class Adjoint:
     def set_initial(self) -> None:
         check(self.lib.psim_adj_set_initial(self._adj), "psim_adj_set_initial")
 
+    def set_cutoff_window(self, w: float) -> None:
+        """w > 0 adds the pairs that cross the cutoff to dL/dcutoff, linearised
+        over [cutoff - w, cutoff + w] (the force jumps there); 0 = pathwise."""
+        check(self.lib.psim_adj_set_cutoff_window(self._adj, float(w)),
+              "psim_adj_set_cutoff_window")
+
     def forward(self) -> int:
         """Run the trajectory with the system's current parameters; returns
         the status (0 or a physics error code)."""

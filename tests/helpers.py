@@ -70,6 +70,24 @@ def pair_abs_sum(pos, box, rc, fmag):
     return out
 
 
+def pair_abs_sum_all(pos, box, rc, fmag, chunk=8_000_000):
+    """sum_j |f(r_ij)| for every particle of a large system: all pairs i < j
+    within rc at once (cKDTree.query_pairs), their magnitudes added to both
+    ends with bincount, in chunks."""
+    pos = np.asarray(pos)
+    n = len(pos)
+    pairs = cKDTree(np.mod(pos, box), boxsize=box).query_pairs(rc, output_type="ndarray")
+    out = np.zeros(n)
+    for a in range(0, len(pairs), chunk):
+        p = pairs[a:a + chunk]
+        d = pos[p[:, 1]] - pos[p[:, 0]]
+        d -= box * np.round(d / box)
+        w = np.abs(fmag(np.sqrt((d * d).sum(axis=1))))
+        out += np.bincount(p[:, 0], weights=w, minlength=n)
+        out += np.bincount(p[:, 1], weights=w, minlength=n)
+    return out
+
+
 def pair_abs_sum_subset(pos, box, rc, fmag, idx):
     """sum_j |f(r_ij)| for the particles idx only (large systems)."""
     pos = np.asarray(pos)

@@ -6,6 +6,8 @@ GPU and oracle trajectories agree to rounding, so their adjoints agree to
 1e-8 relative.  The finite differences use a relative parameter step small
 enough that no pair crosses the cutoff between the two runs (the adjoint is
 pathwise) and are compared at 1e-4 relative, the bar for adjoint gradients.
+The cutoff window's crossing term is compared with the oracle's (whose own
+test pins it against central differences).
 """
 import numpy as np
 import pytest
@@ -89,6 +91,47 @@ def test_adjoint_matches_oracle_adjoint(oracle, case, path):
     for k, name in enumerate(ps.engine.ADJOINT_PARAMS):
         assert g[name] == pytest.approx(dth[k], rel=1e-8, abs=1e-14), name
     assert np.abs(g["gx0"] - gx0).max() <= 1e-8 * np.abs(gx0).max()
+
+
+@pytest.mark.parametrize("path", ["sparse", "dense"])
+def test_cutoff_window_matches_oracle(oracle, path):
+    """The cutoff-crossing term (window 0.01, about 14 pairs per step at this
+    size) against the oracle's: same trajectory to rounding, so the same
+    window pairs and the same sum to 1e-8."""
+    case, win = "ao", 0.01
+    kw, n, L, S, K, every, seed, _ = CASES[case]
+    print(f"adjoint window path={path}", flush=True)
+    sysg, pos, vel = make(case, path)
+    w = np.random.default_rng(98).standard_normal((n, 3))
+    with sysg, ps.Adjoint(sysg, K, every) as adj:
+        adj.set_cutoff_window(win)
+        assert adj.forward() == 0
+        g = adj.backward(w)
+        adj.set_cutoff_window(0.0)
+        g0 = adj.backward(w)
+    c = oracle.gb_coeffs(1.0, 1.0, 1.0, 1e-3)
+    p0 = np.where(np.abs(pos) > L / 2, np.where(pos < 0, -L, L), 0.0)
+    st, dth, _, _ = oracle.adjoint(oracle_params(kw), c, dict(pos=pos, pos0=p0, vel=vel), K, L,
+                                   S, L / S, w, vel_freq=4, seed=seed, cut_window=win)
+    assert st == 0
+    for k, name in enumerate(ps.engine.ADJOINT_PARAMS):
+        assert g[name] == pytest.approx(dth[k], rel=1e-8, abs=1e-14), name
+    # only the cutoff derivative moves, and by much more than the tolerance
+    assert abs(g["cutoff"] - g0["cutoff"]) > 1e-3 * abs(g0["cutoff"])
+    assert all(g[k] == g0[k] for k in ps.engine.ADJOINT_PARAMS if k != "cutoff")
+
+
+def test_cutoff_window_beyond_the_cell_edge_is_refused():
+    sysg, _, _ = make("ao")      # cutoff 1.12, cell edge 16/13 = 1.2308
+    with sysg, ps.Adjoint(sysg, 3, 2) as adj:
+        assert adj.forward() == 0
+        adj.set_cutoff_window(0.2)
+        with pytest.raises(ps.PsimError):
+            adj.backward(np.zeros((sysg.n, 3)))
+        adj.set_cutoff_window(0.1)
+        adj.backward(np.zeros((sysg.n, 3)))
+        with pytest.raises(ps.PsimError):
+            adj.set_cutoff_window(-1.0)
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():

@@ -235,6 +235,9 @@ def test_error_codes():
         assert sysg.forces_now()[0] == 7705
 
 
+_C3_SCALE = {}
+
+
 @pytest.mark.parametrize("path", PATHS)
 def test_full_size_c3_properties(oracle, path):
     """N = 2,097,152 Yukawa + LJ at phi 0.30: a few steps, then size-independent
@@ -262,10 +265,16 @@ def test_full_size_c3_properties(oracle, path):
     sto, fo, _ = oracle.forces(fp, out["pos"], geom.cells_per_dim, geom.cell_size, geom.box,
                                half_uv=geom.box / 2)
     assert sto == 0
-    from helpers import pair_abs_sum_subset
-    idx = np.random.default_rng(0).choice(n, 20000, replace=False)
-    denom = pair_abs_sum_subset(out["pos"], geom.box, fp.cutoff, pair_fmag(fp), idx)
-    err = force_rel_err(f[idx], fo[idx], denom)
-    assert err.max() <= FORCE_TOL, f"max {err.max():.3g}"
-    # every particle: absolute error against the largest force
+    # every particle, relative to max(|F_i|, sum_j |f_ij|): a pair dropped or
+    # added at the cutoff (|f| ~ 1.6e-3 kT/sigma at r = 3) shows
+    # The scale is computed once and reused by the other kernel paths: their
+    # positions agree to round-off, far below what moves a pair sum.
+    from helpers import pair_abs_sum_all
+    if _C3_SCALE.get("pos") is None or np.abs(_C3_SCALE["pos"] - out["pos"]).max() > 1e-9:
+        print("c3: pair sums", flush=True)
+        _C3_SCALE["pos"] = out["pos"]
+        _C3_SCALE["denom"] = pair_abs_sum_all(out["pos"], geom.box, fp.cutoff, pair_fmag(fp))
+    denom = _C3_SCALE["denom"]
+    err = force_rel_err(f, fo, denom)
+    assert err.max() <= FORCE_TOL, f"max {err.max():.3g} at {int(err.argmax())}"
     assert np.abs(f - fo).max() <= FORCE_TOL * np.abs(fo).max()

@@ -118,3 +118,81 @@ def test_adjoint_matches_central_differences(oracle, case):
         d -= L * np.round(d / L)
         fd = float(np.sum(w * d)) / 2e-6
         assert gx0[i, a] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+
+
+def _pair_distances(p, L):
+    d = p[:, None, :] - p[None, :, :]
+    d -= L * np.round(d / L)
+    r = np.sqrt((d * d).sum(axis=2))
+    iu = np.triu_indices(len(p), 1)
+    return r[iu]
+
+
+def test_cutoff_window_matches_central_differences(oracle):
+    """The cutoff window's boundary term, against the secant over
+    [c - w, c + w] when exactly one pair of the trajectory lies in the window
+    (at the first step, at r = c - w/3) and every other pair distance of
+    every step stays 1e-4 from c, so the kick of that one pair's force (a
+    position change of order dt^2 F = 1e-6 per step) toggles nothing else.
+    The secant holds that pair's force jump propagated through the later
+    steps, with the pair's force taken at the coefficients of c + w:
+    f(r; c + w) = f(r; c) + w df/dc, so
+        secant = window estimate - P / 2   (to O(w^2)),
+    P being the pair's own pathwise term (in the window estimate with weight
+    1), measured as the pathwise derivative at c minus that at r - w/3,
+    where the pair is outside.  (Over pairs spread evenly across the window
+    this term averages out.)"""
+    n, L, S, K, w = 100, 12.0, 10, 3, 1e-5
+    kw = dict(mask=PO_AO, cutoff=1.1, kT=1.0, ao_well_depth=0.4)
+    pos = rsa_positions(n, L, 21)
+    rng = np.random.default_rng(5)
+    vel = rng.standard_normal((n, 3))
+    gw = rng.standard_normal((n, 3))
+    c = oracle.gb_coeffs(1.0, 1.0, 1.0, 1e-3)
+
+    def traj(kwx):
+        st = dict(pos=pos.copy(), pos0=_fresh_pos0(pos, L), vel=vel.copy(),
+                  frc=np.zeros((n, 3)), frc0=np.zeros((n, 3)), memx=np.zeros((n, 3)),
+                  time=0.0)
+        xs = [st["pos"].copy()]
+        for _ in range(K):
+            s, _ = oracle.run(_fp(kwx), c, st, 1, L, S, L / S, seed=17)
+            assert s == 0
+            xs.append(st["pos"].copy())
+        return xs
+
+    # c = r + w/3 for a first-step pair at r in [1.03, 1.17] (x_0 does not
+    # depend on c) whose trajectory at that c keeps the margin everywhere else
+    d0 = _pair_distances(pos, L)
+    cut = None
+    for k in np.argsort(np.abs(d0 - 1.1)):
+        cand = float(d0[k] + w / 3)
+        if abs(cand - 1.1) > 0.07:
+            break
+        if np.abs(np.delete(d0, k) - cand).min() <= 1e-4:
+            continue
+        kc = dict(kw, cutoff=cand)
+        if all(np.abs(_pair_distances(x, L) - cand).min() > 1e-4 for x in traj(kc)[1:K]):
+            cut = cand
+            break
+    assert cut is not None, "no isolated pair near the cutoff"
+    kw["cutoff"] = cut
+    state = dict(pos=pos, pos0=_fresh_pos0(pos, L), vel=vel)
+    s, dth0, _, _ = oracle.adjoint(_fp(kw), c, state, K, L, S, L / S, gw, seed=17)
+    s1, dthw, _, _ = oracle.adjoint(_fp(kw), c, state, K, L, S, L / S, gw, seed=17,
+                                    cut_window=w)
+    assert s == 0 and s1 == 0
+    kp, km = dict(kw), dict(kw)
+    kp["cutoff"] += w
+    km["cutoff"] -= w
+    d = traj(kp)[K] - traj(km)[K]
+    d -= L * np.round(d / L)
+    fd = float(np.sum(gw * d)) / (2 * w)
+    # the jump dominates: the pathwise derivative alone misses it
+    assert abs(fd - dth0[1]) > abs(dth0[1])
+    sx, dthx, _, _ = oracle.adjoint(_fp(dict(kw, cutoff=cut - 2 * w / 3)), c, state, K, L, S,
+                                    L / S, gw, seed=17)
+    assert sx == 0
+    est = dthw[1] - 0.5 * (dth0[1] - dthx[1])
+    assert est == pytest.approx(fd, rel=1e-4), (dthw[1], est, fd, dth0[1])
+    assert np.array_equal(dthw[[0, 2, 3, 4]], dth0[[0, 2, 3, 4]])
