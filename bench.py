@@ -244,8 +244,11 @@ def valu_from_profiles(prof, pair_ms):
             "insts_per_particle_step_lanes": insts * 64 / N,
             "achieved_lane_ops_per_s": rate, "peak": VALU_PEAK_LANE_OPS,
             "frac": rate / VALU_PEAK_LANE_OPS,
-            "note": "SQ_INSTS_VALU x 64 counts issued lanes incl. masked ones; a frac "
-                    "near 1 means VALU-issue bound"}
+            "note": "SQ_INSTS_VALU x 64 (issued lanes incl. masked ones) from the committed "
+                    "pmc run over this run's launch time, against 1024 SIMDs x 16 lanes x "
+                    "2.4 GHz; the pmc run's own launches (slower under the profiler) put the "
+                    "kernel at 98% of that model, so a frac near or slightly above 1 reads as "
+                    "VALU-issue bound"}
 
 
 def small_leg(name: str, steps: int, warmup: int = 10, eager: bool = False) -> dict:

@@ -56,12 +56,11 @@ struct DenseSharedT {
 };
 using DenseShared = DenseSharedT<kDenseThreads>;
 // waves per SIMD for the kernels of force mask `mask` and compile-time LJ
-// exponent `ljn` (0: ljNum read at run time): LJ with AO, or Calibration with
-// LJ and a run-time exponent, need more than 72 VGPRs (6 waves)
+// exponent `ljn` (0: ljNum read at run time): LJ with AO, or LJ with a
+// run-time exponent, need more than 72 VGPRs (6 waves)
 constexpr int dense_waves(int mask, int ljn = 0)
 {
-    const bool wide = ((mask & F_LJ) && (mask & F_AO)) ||
-                      ((mask & F_CAL) && (mask & F_LJ) && ljn == 0);
+    const bool wide = ((mask & F_LJ) && (mask & F_AO)) || ((mask & F_LJ) && ljn == 0);
     return wide ? (PSIM_DENSE_WAVES < 6 ? PSIM_DENSE_WAVES : 6) : PSIM_DENSE_WAVES;
 }
 // the workgroups that PSIM_DENSE_WAVES waves per SIMD make resident on one CU
@@ -200,9 +199,10 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
     const double ov = p.ov_factor * size;
     const double ov2_hi = ov * ov * (1.0 + p.ov_band);
     // FP32 cull: coordinates below 2 cells in magnitude carry errors below
-    // 2.5e-7 cell each, so r^2 is within 2e-6 cell^2 of the FP64 value;
-    // the threshold is widened by ten times that
-    const float rc2_f = (float)(p.rc2_hi + 2e-5 * g.cell * g.cell);
+    // 2.5e-7 cell each, so r^2 is within 2e-6 cell^2 of the FP64 value, and
+    // r^2 - rc^2 evaluated as three FMAs adds at most 1.5 ulp of 12 cell^2
+    // (2e-6 cell^2); the threshold is widened by 2e-5 cell^2
+    const float nrc2_f = -(float)(p.rc2_hi + 2e-5 * g.cell * g.cell);
 
     for (int i0 = 0; i0 < ni; i0 += T) {
         const int nic = min(T, ni - i0);
@@ -242,27 +242,36 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
             }
             __syncthreads();
             if (act) {
-                // self is accepted by the cull (r^2 = 0) and skipped in the drain
+                // self is accepted by the cull (r^2 = 0) and skipped in the drain.
+                // Candidate k of this lane is gg + k G; every lane has at
+                // least kmin = mc / G of them, so the unrolled part of each
+                // batch runs a wave-uniform number of quads.
+                const int kmin = __builtin_amdgcn_readfirstlane(mc) / G;
                 int j = gg;
-                while (j < mc) {
+                for (int b = 0; j < mc; b += 32) {
                     const int j0 = j;
                     unsigned m = 0;
                     int n = 0;
-                    for (; n <= 28 && j + 3 * G < mc; n += 4, j += 4 * G) {
+                    const int nq = __builtin_amdgcn_readfirstlane(min(8, max(0, kmin - b) >> 2));
+                    int jq = j;
+                    for (int q = 0; q < nq; ++q, jq += 4 * G) {
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
-                            const int jj = j + u * G;
+                            const int jj = jq + u * G;
                             const float dx = sh.u.cf.x[jj] - xf, dy = sh.u.cf.y[jj] - yf,
                                         dz = sh.u.cf.z[jj] - zf;
-                            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                            m = (m << 1) | (r2 < rc2_f ? 1u : 0u);
+                            // r^2 - rc^2: its sign bit is the survivor bit
+                            const float d = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, nrc2_f)));
+                            m = __builtin_amdgcn_alignbit(m, __float_as_uint(d), 31);
                         }
                     }
+                    n = 4 * nq;
+                    j += 4 * G * nq;
                     for (; n < 32 && j < mc; ++n, j += G) {
                         const float dx = sh.u.cf.x[j] - xf, dy = sh.u.cf.y[j] - yf,
                                     dz = sh.u.cf.z[j] - zf;
-                        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        m = (m << 1) | (r2 < rc2_f ? 1u : 0u);
+                        const float d = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, nrc2_f)));
+                        m = __builtin_amdgcn_alignbit(m, __float_as_uint(d), 31);
                     }
                     dense_drain_mask<MASK, T, LJN>(g, p, sh, pos, xi, yi, zi, size, ov2_hi,
                                                    hi - s0, m, n, j0, G, fx, fy, fz, err, id_i,
