@@ -57,11 +57,13 @@ struct DenseSharedT {
 using DenseShared = DenseSharedT<kDenseThreads>;
 // waves per SIMD for the kernels of force mask `mask` and compile-time LJ
 // exponent `ljn` (0: ljNum read at run time): LJ with AO, or LJ with a
-// run-time exponent, need more than 72 VGPRs (6 waves)
+// run-time exponent, need more than 72 VGPRs (6 waves), and LJ with AO and a
+// run-time exponent more than 80 (5 waves)
 constexpr int dense_waves(int mask, int ljn = 0)
 {
-    const bool wide = ((mask & F_LJ) && (mask & F_AO)) || ((mask & F_LJ) && ljn == 0);
-    return wide ? (PSIM_DENSE_WAVES < 6 ? PSIM_DENSE_WAVES : 6) : PSIM_DENSE_WAVES;
+    const bool lj_ao = (mask & F_LJ) && (mask & F_AO);
+    const int w = (lj_ao && ljn == 0) ? 5 : (lj_ao || ((mask & F_LJ) && ljn == 0)) ? 6 : 7;
+    return PSIM_DENSE_WAVES < w ? PSIM_DENSE_WAVES : w;
 }
 // the workgroups that PSIM_DENSE_WAVES waves per SIMD make resident on one CU
 // (4 SIMDs) must also fit its 160 KiB of LDS
@@ -76,19 +78,35 @@ __device__ inline int xcd_cell(int b, int per_xcd)
     return (b & 7) * per_xcd + (b >> 3);
 }
 
+// global double at a 32-bit byte offset from base
+__device__ __forceinline__ double ld_off(const double *__restrict__ base, unsigned off)
+{
+    return *(const double *)((const char *)base + off);
+}
+
+// element of a staged array at byte offset off (the LDS address is then the
+// offset plus a constant, with no index scaling)
+__device__ __forceinline__ double lds_at(const double *a, int off)
+{
+    return *(const double *)((const char *)a + off);
+}
+
 // Pair force of staged candidate j on the lane's home particle (FP64; the
 // exact pbcDist recheck for survivors inside the guard band).
+// j8 = 8 j, the byte offset of candidate j in the staged double arrays.
 template <int MASK, int T, int LJN, class ST>
 __device__ __forceinline__ void dense_pair(const Geo &g, const Pot &p,
                                            const DenseSharedT<T> &sh,
                                            const double *__restrict__ pos, double xi, double yi,
-                                           double zi, double size, double ov2_hi, int j,
+                                           double zi, double size, double ov2_hi, int j8,
                                            double &fx, double &fy, double &fz, DevErr *err,
                                            int id_i, const int *__restrict__ id, ST &st)
 {
     const int N = g.n;
-    const double dx = sh.sx[j] - xi, dy = sh.sy[j] - yi, dz = sh.sz[j] - zi;
+    const double dx = lds_at(sh.sx, j8) - xi, dy = lds_at(sh.sy, j8) - yi,
+                 dz = lds_at(sh.sz, j8) - zi;
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const int j = j8 >> 3;
     if (r2 >= p.rc2_lo || r2 < ov2_hi) {
         const int sj = sh.sslot[j];
         const double r2x = pbc_dist_ref(xi, yi, zi, pos[sj], pos[N + sj], pos[2 * N + sj],
@@ -123,7 +141,8 @@ __device__ __forceinline__ int mad_i24(int b, int c_uniform, int a)
 // Survivors of one batch of n <= 32 candidates j0, j0 + G, ...: bit
 // (n - 1 - k) of m is set when candidate j0 + k G passed the cull.  The
 // lowest set bit is taken first (m &= m - 1), so the batch is evaluated
-// from its last candidate down: bit c is candidate j0 + (n - 1 - c) G.
+// from its last candidate down: bit c is candidate j0 + (n - 1 - c) G, at
+// byte offset 8 (j0 + (n - 1 - c) G) in the staged arrays.
 template <int MASK, int T, int LJN, class ST>
 __device__ void dense_drain_mask(const Geo &g, const Pot &p, const DenseSharedT<T> &sh,
                                  const double *__restrict__ pos, double xi, double yi,
@@ -131,13 +150,13 @@ __device__ void dense_drain_mask(const Geo &g, const Pot &p, const DenseSharedT<
                                  int n, int j0, int G, double &fx, double &fy, double &fz,
                                  DevErr *err, int id_i, const int *__restrict__ id, ST &st)
 {
-    const int jtop = j0 + (n - 1) * G;
+    const int jtop8 = 8 * (j0 + (n - 1) * G), self8 = 8 * self;
     while (m) {
         const int c = __builtin_ctz(m);
         m &= m - 1u;
-        const int j = mad_i24(c, -G, jtop);
-        if (j == self) continue;
-        dense_pair<MASK, T, LJN>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, j, fx, fy, fz, err,
+        const int j8 = mad_i24(c, -8 * G, jtop8);
+        if (j8 == self8) continue;
+        dense_pair<MASK, T, LJN>(g, p, sh, pos, xi, yi, zi, size, ov2_hi, j8, fx, fy, fz, err,
                                  id_i, id, st);
     }
 }
@@ -229,8 +248,10 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
                 const double ex = sh.shx[q], ey = sh.shy[q], ez = sh.shz[q];
                 for (int t = t_lo + u; t < t_hi; t += kPer) {
                     const int slot = base + t, k = t - s0;
-                    const double x = pos[slot] + ex, y = pos[N + slot] + ey,
-                                 z = pos[2 * N + slot] + ez;
+                    // 32-bit byte offsets (3 N doubles stay below 4 GiB)
+                    const unsigned o = 8u * (unsigned)slot, o3 = 8u * (unsigned)N;
+                    const double x = ld_off(pos, o) + ex, y = ld_off(pos, o + o3) + ey,
+                                 z = ld_off(pos, o + 2u * o3) + ez;
                     sh.sx[k] = x;
                     sh.sy[k] = y;
                     sh.sz[k] = z;
@@ -298,20 +319,47 @@ __device__ void dense_cell_forces(const Geo &g, const Pot &p, double radius,
             __syncthreads();
             continue;
         }
-        // ---- sum the G group partials of each home particle
+        // ---- sum the G group partials of each home particle.  Up to 64
+        // home particles: P = 64 / nic (at most 4) lanes of wave 0 per
+        // particle each add every P-th partial, then the particle's lane
+        // adds the P sums; beyond that, the particle's lane adds all G.
         __syncthreads();   // the cull's copy is no longer read
         sh.u.red[0][lane] = fx;
         sh.u.red[1][lane] = fy;
         sh.u.red[2][lane] = fz;
         __syncthreads();
-        if (lane < nic) {
-            double Fx = 0.0, Fy = 0.0, Fz = 0.0;
-            for (int q = 0; q < G; ++q) {
-                Fx += sh.u.red[0][lane + q * nic];
-                Fy += sh.u.red[1][lane + q * nic];
-                Fz += sh.u.red[2][lane + q * nic];
+        if (nic > 64) {
+            if (lane < nic) {
+                double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+                for (int q = 0; q < G; ++q) {
+                    Fx += sh.u.red[0][lane + q * nic];
+                    Fy += sh.u.red[1][lane + q * nic];
+                    Fz += sh.u.red[2][lane + q * nic];
+                }
+                epi(i0 + lane, Fx, Fy, Fz);
             }
-            epi(i0 + lane, Fx, Fy, Fz);
+        } else if (lane < 64) {
+            const int P = min(4, 64 / nic);
+            double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+            if (lane < P * nic) {
+                const int gi2 = lane % nic;
+                for (int q = lane / nic; q < G; q += P) {
+                    Fx += sh.u.red[0][gi2 + q * nic];
+                    Fy += sh.u.red[1][gi2 + q * nic];
+                    Fz += sh.u.red[2][gi2 + q * nic];
+                }
+            }
+            for (int k = 1; k < P; ++k) {
+                const int src = lane + k * nic;   // below 64 for lane < nic
+                const double ax = __shfl(Fx, src, 64), ay = __shfl(Fy, src, 64),
+                             az = __shfl(Fz, src, 64);
+                if (lane < nic) {
+                    Fx += ax;
+                    Fy += ay;
+                    Fz += az;
+                }
+            }
+            if (lane < nic) epi(i0 + lane, Fx, Fy, Fz);
         }
         __syncthreads();
     }

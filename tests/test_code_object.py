@@ -100,8 +100,8 @@ def test_dense_kernels_fit_their_waves(kernels, kernel):
             continue
         seen += 1
         mask, ljn = int(m.group(1)), int(m.group(2))
-        wide = (mask & 2 and mask & 4) or (mask & 2 and ljn == 0)
-        waves = 6 if wide else 7
+        lj_ao = mask & 2 and mask & 4
+        waves = 5 if (lj_ao and ljn == 0) else 6 if (lj_ao or (mask & 2 and ljn == 0)) else 7
         assert v["vgpr_count"] <= (512 // waves) // 8 * 8, (name, v)
         assert waves * v["group_segment_fixed_size"] <= 160 * 1024, (name, v)
     assert seen >= 16
