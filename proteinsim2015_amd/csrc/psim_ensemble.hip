@@ -56,17 +56,19 @@ __device__ __forceinline__ int replica_of(const int *__restrict__ p0, int nrep, 
     return lo;
 }
 
-// Thread-per-particle step: workgroup `it` of the launch covers slots
+// Thread-per-particle pair forces: workgroup `it` of the launch covers slots
 // p0[r] + (it - blk0[r]) BLOCK .. of replica r (blk0 = prefix of the
 // replicas' workgroup counts), so a workgroup never spans two replicas and
 // the replica's geometry and constants are the same for every lane (scalar
-// loads, no per-lane replica search).
+// loads, no per-lane replica search).  The forces go to nfrc, where the
+// integrator (k_ens_integ, the same launch shape) reads them: as two
+// kernels the 5.25 M-particle ensemble step takes 0.927 ms against 0.945
+// with the integrator in this kernel's epilogue.
 template <int MASK, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_ens_step(const RepGeo *__restrict__ reps,
                                                     const int *__restrict__ p0,
                                                     const int *__restrict__ blk0, int nrep,
-                                                    int nblk, int per_xcd, int ntot, GB gb,
-                                                    StepArgs a)
+                                                    int nblk, int per_xcd, int ntot, StepArgs a)
 {
     __shared__ unsigned hits[kHitCap * BLOCK];
     const int it = xcd_cell(blockIdx.x, per_xcd);
@@ -81,8 +83,6 @@ __global__ __launch_bounds__(BLOCK) void k_ens_step(const RepGeo *__restrict__ r
     const double radius = reps[r].radius;
     const int N = ntot;
     const double xi = a.pos[s], yi = a.pos[N + s], zi = a.pos[2 * N + s];
-    // the integrator's index loads, in flight during the force loop
-    const int o = a.id[s], ps = a.prev ? a.prev[s] : s;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     SoAPos src{a.pos, N, radius};
     StartTab tab{a.cstart};
@@ -93,7 +93,26 @@ __global__ __launch_bounds__(BLOCK) void k_ens_step(const RepGeo *__restrict__ r
     else
         gather_force<MASK, false, BLOCK>(g, p, src, tab, s, xi, yi, zi, radius,
                                          hits + threadIdx.x, fx, fy, fz, a.err, orig);
-    integrate_one<true>(g, gb, a, s, fx, fy, fz, o, ps);
+    a.nfrc[s] = fx;
+    a.nfrc[N + s] = fy;
+    a.nfrc[2 * N + s] = fz;
+}
+
+// The G&B step of every particle with the forces k_ens_step left in nfrc.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_ens_integ(const RepGeo *__restrict__ reps,
+                                                     const int *__restrict__ p0,
+                                                     const int *__restrict__ blk0, int nrep,
+                                                     int nblk, int per_xcd, int ntot, GB gb,
+                                                     StepArgs a)
+{
+    const int it = xcd_cell(blockIdx.x, per_xcd);
+    if (it >= nblk) return;
+    const int r = replica_of(blk0, nrep, it);
+    const int s = p0[r] + (it - blk0[r]) * BLOCK + (int)threadIdx.x;
+    if (s >= p0[r + 1]) return;
+    const int N = ntot;
+    integrate_one<false>(reps[r].g, gb, a, s, a.nfrc[s], a.nfrc[N + s], a.nfrc[2 * N + s]);
 }
 
 // Brick kernel: workgroup `it` of the launch is brick it - item0[r] of
@@ -294,6 +313,8 @@ void launch_ens_pair(psim_ens *e, const StepArgs &a)
 {
     const int per = (e->nblk + 7) / 8;
     k_ens_step<MASK, BLOCK><<<8 * per, BLOCK, 0, e->stream>>>(
+        e->reps, e->p0, e->blk0, e->nrep, e->nblk, per, e->ntot, a);
+    k_ens_integ<BLOCK><<<8 * per, BLOCK, 0, e->stream>>>(
         e->reps, e->p0, e->blk0, e->nrep, e->nblk, per, e->ntot, e->gb, a);
 }
 
