@@ -400,6 +400,14 @@ void psim_adj_destroy(psim_adjoint *adj);
  * half-width.  cutOff + w must not exceed the cell edge (else backward
  * returns PSIM_E_ARG). */
 int psim_adj_set_cutoff_window(psim_adjoint *adj, double w);
+/* Record-all mode (on != 0): the forward pass records every state of the
+ * trajectory (positions, ids, cell table: about (32 N + 4 ncell) bytes per
+ * step, so K (32 N + 4 ncell) in all), and the backward pass recomputes
+ * nothing; off (the default) keeps checkpoint_every-step segments that the
+ * backward pass recomputes.  The gradients are bit-identical either way.
+ * Takes effect at the next psim_adj_forward; PSIM_E_NOMEM (the object stays
+ * in checkpoint mode) when the records do not fit. */
+int psim_adj_set_record_all(psim_adjoint *adj, int on);
 /* Re-take the start state from the context's current state. */
 int psim_adj_set_initial(psim_adjoint *adj);
 /* Run the trajectory from the start state with the context's current

@@ -362,9 +362,13 @@ struct psim_adjoint {
     std::vector<Meta> meta;        // ncp + 1 (the last = final state)
     double *cp_d = nullptr;        // (ncp + 1) x 15N: pos, pos0, frc, mem, vel
     int *cp_i = nullptr;           // (ncp + 1) x (N + ncell + 1): id, cstart
-    double *rec_pos = nullptr;     // every x 3N
-    int *rec_i = nullptr;          // every x (N + ncell + 1)
-    int *rec_inv = nullptr;        // every x N: original index -> slot of each record
+    // records of the states of one segment (every of them), or of the whole
+    // trajectory (K, record_all: written by the forward pass itself)
+    double *rec_pos = nullptr;     // nrec x 3N
+    int *rec_i = nullptr;          // nrec x (N + ncell + 1)
+    int *rec_inv = nullptr;        // nrec x N: original index -> slot of each record
+    int nrec = 0;
+    bool record_all = false;
     double *a[3] = {nullptr, nullptr, nullptr};
     double *lam[2] = {nullptr, nullptr};
     double *tpart = nullptr, *acc = nullptr;
@@ -396,6 +400,54 @@ void add_seg(CopySegs &cs, const void *src, void *dst, size_t bytes)
     cs.dst[cs.nseg] = (unsigned *)dst;
     cs.end[cs.nseg] = (cs.nseg ? cs.end[cs.nseg - 1] : 0) + (long long)(bytes / 4);
     ++cs.nseg;
+}
+
+// Record buffers for nrec states (the previous ones are freed).
+int alloc_records(psim_adjoint *A, int nrec)
+{
+    psim_ctx *c = A->c;
+    void *old[] = {A->rec_pos, A->rec_i, A->rec_inv};
+    for (void *b : old)
+        if (b) (void)hipFree(b);
+    A->rec_pos = nullptr;
+    A->rec_i = nullptr;
+    A->rec_inv = nullptr;
+    A->nrec = 0;
+    int rc = dalloc(&A->rec_pos, (size_t)nrec * 3 * (size_t)c->n);
+    if (!rc) rc = dalloc(&A->rec_i, (size_t)nrec * cp_stride_i(c));
+    if (!rc) rc = dalloc(&A->rec_inv, (size_t)nrec * c->n);
+    if (!rc) A->nrec = nrec;
+    return rc;
+}
+
+// Record r of the records: the state's positions, ids, cell table and
+// inverse slot map.
+psim_ctx::Record record_at(psim_adjoint *A, int r)
+{
+    psim_ctx *c = A->c;
+    const size_t si = cp_stride_i(c);
+    psim_ctx::Record R{};
+    R.pos = A->rec_pos + (size_t)r * 3 * (size_t)c->n;
+    R.id = A->rec_i + (size_t)r * si;
+    R.cstart = A->rec_i + (size_t)r * si + c->n;
+    R.inv = A->rec_inv + (size_t)r * c->n;
+    return R;
+}
+
+// The context's current state (settled: current slot order) as record r.
+int record_now(psim_adjoint *A, int r)
+{
+    psim_ctx *c = A->c;
+    const psim_ctx::Record R = record_at(A, r);
+    CopySegs cs{};
+    add_seg(cs, c->pos, R.pos, sizeof(double) * 3 * (size_t)c->n);
+    add_seg(cs, c->id, R.id, sizeof(int) * c->n);
+    add_seg(cs, c->cstart, R.cstart, sizeof(int) * (c->ncell + 1));
+    int rc = copy_segs(c, cs);
+    if (rc) return rc;
+    k_inv<<<blocks_for(c->n, 256), 256, 0, c->stream>>>(c->n, R.id, R.inv);
+    CK(hipGetLastError());
+    return 0;
 }
 
 int save_cp(psim_adjoint *A, int k)
@@ -554,9 +606,7 @@ int psim_adj_create(psim_ctx *c, int nsteps, int checkpoint_every, psim_adjoint 
     int rc = 0;
     if (!rc) rc = dalloc(&A->cp_d, (size_t)(A->ncp + 1) * cp_stride_d(c));
     if (!rc) rc = dalloc(&A->cp_i, (size_t)(A->ncp + 1) * cp_stride_i(c));
-    if (!rc) rc = dalloc(&A->rec_pos, (size_t)A->every * n3);
-    if (!rc) rc = dalloc(&A->rec_i, (size_t)A->every * cp_stride_i(c));
-    if (!rc) rc = dalloc(&A->rec_inv, (size_t)A->every * c->n);
+    if (!rc) rc = alloc_records(A, A->every);
     for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(&A->a[k], n3);
     for (int k = 0; k < 2 && !rc; ++k) rc = dalloc(&A->lam[k], n3);
     if (!rc) rc = dalloc(&A->tpart, (size_t)nsteps * nb * kNTheta);
@@ -582,6 +632,26 @@ void psim_adj_destroy(psim_adjoint *A)
     delete A;
 }
 
+int psim_adj_set_record_all(psim_adjoint *A, int on)
+{
+    if (!A) return PSIM_E_ARG;
+    const int want = on ? A->K : A->every;
+    if (A->nrec != want) {
+        CK(hipStreamSynchronize(A->c->stream));
+        const int rc = alloc_records(A, want);
+        if (rc) {
+            // back to the segment records (smaller) so the object stays usable
+            A->record_all = false;
+            if (alloc_records(A, A->every)) A->nrec = 0;
+            A->have_forward = false;
+            return rc;
+        }
+    }
+    A->record_all = on != 0;
+    A->have_forward = false;   // the records are made by the next forward
+    return 0;
+}
+
 int psim_adj_set_cutoff_window(psim_adjoint *A, double w)
 {
     if (!A || !(w >= 0.0) || !std::isfinite(w)) return PSIM_E_ARG;
@@ -602,6 +672,7 @@ int psim_adj_set_initial(psim_adjoint *A)
 int psim_adj_forward(psim_adjoint *A)
 {
     if (!A) return PSIM_E_ARG;
+    if (A->nrec == 0) return PSIM_E_NOMEM;
     psim_ctx *c = A->c;
     TrackerPause pause(c);
     int rc = restore_cp(A, 0);
@@ -611,11 +682,24 @@ int psim_adj_forward(psim_adjoint *A)
         A->first[n] = (t == 0) ? 1 : 0;   // brownianIntegrator.cpp:183
         t += c->prm.dt;
     }
-    for (int k = 0; k < A->ncp; ++k) {
-        if (k > 0 && (rc = save_cp(A, k))) return rc;
-        const int n0 = k * A->every;
-        const int n1 = n0 + A->every < A->K ? n0 + A->every : A->K;
-        if ((rc = eng::run_steps(c, n1 - n0, nullptr))) return rc;
+    if (A->record_all) {
+        // every state of the trajectory recorded on the way (the cell
+        // rebuild of each step writes the next one): no checkpoints between
+        // the start and the end, and no recomputation in the backward pass
+        if ((rc = record_now(A, 0))) return rc;
+        for (int n = 0; n < A->K; ++n) {
+            if (n + 1 < A->K) c->rec = record_at(A, n + 1);
+            rc = eng::run_steps(c, 1, nullptr);
+            c->rec = psim_ctx::Record{};
+            if (rc) return rc;
+        }
+    } else {
+        for (int k = 0; k < A->ncp; ++k) {
+            if (k > 0 && (rc = save_cp(A, k))) return rc;
+            const int n0 = k * A->every;
+            const int n1 = n0 + A->every < A->K ? n0 + A->every : A->K;
+            if ((rc = eng::run_steps(c, n1 - n0, nullptr))) return rc;
+        }
     }
     if ((rc = save_cp(A, A->ncp))) return rc;
     int code = 0;
@@ -630,6 +714,7 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
     if (!A || !gx || !dtheta) return PSIM_E_ARG;
     psim_ctx *c = A->c;
     if (!A->have_forward || !same_forces(A->prm_used, c->prm)) return PSIM_E_ARG;
+    if (A->nrec == 0) return PSIM_E_NOMEM;
     // the window's pairs must lie in the 27 cells the walk visits
     if (A->win > 0.0 && c->prm.cutoff + A->win > c->geo.cell) return PSIM_E_ARG;
     TrackerPause pause(c);
@@ -659,11 +744,14 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
     double *lam = A->lam[0], *lam_prev = A->lam[1];
     const size_t si = cp_stride_i(c);
     int rc = 0;
-    for (int k = A->ncp - 1; k >= 0; --k) {
-        const int n0 = k * A->every;
-        const int n1 = n0 + A->every < A->K ? n0 + A->every : A->K;
-        if ((rc = restore_cp(A, k))) return rc;
-        {
+    // with every state recorded by the forward pass the trajectory is one
+    // segment with nothing to recompute
+    const int nseg = A->record_all ? 1 : A->ncp;
+    for (int k = nseg - 1; k >= 0; --k) {
+        const int n0 = A->record_all ? 0 : k * A->every;
+        const int n1 = A->record_all ? A->K : (n0 + A->every < A->K ? n0 + A->every : A->K);
+        if (!A->record_all) {
+            if ((rc = restore_cp(A, k))) return rc;
             // the checkpoint's state is record 0; each recomputed step's
             // cell rebuild writes the next record itself
             CopySegs cs{};
@@ -672,16 +760,12 @@ int psim_adj_backward(psim_adjoint *A, const double *gx, double *dtheta, double 
             add_seg(cs, c->cstart, A->rec_i + n, sizeof(int) * (c->ncell + 1));
             if ((rc = copy_segs(c, cs))) return rc;
             k_inv<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, A->rec_i, A->rec_inv);
-        }
-        for (int s = n0; s + 1 < n1; ++s) {
-            const int r = s - n0 + 1;
-            c->rec.pos = A->rec_pos + (size_t)r * n3;
-            c->rec.id = A->rec_i + (size_t)r * si;
-            c->rec.cstart = A->rec_i + (size_t)r * si + n;
-            c->rec.inv = A->rec_inv + (size_t)r * n;
-            rc = eng::run_steps(c, 1, nullptr);
-            c->rec = psim_ctx::Record{};
-            if (rc) return rc;
+            for (int s = n0; s + 1 < n1; ++s) {
+                c->rec = record_at(A, s - n0 + 1);
+                rc = eng::run_steps(c, 1, nullptr);
+                c->rec = psim_ctx::Record{};
+                if (rc) return rc;
+            }
         }
         // lambda_{n1-1} arrives by original index (from the segment after,
         // or the start): into the slot order of the segment's last record

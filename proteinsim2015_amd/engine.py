@@ -566,6 +566,13 @@ class Adjoint:
     def set_initial(self) -> None:
         check(self.lib.psim_adj_set_initial(self._adj), "psim_adj_set_initial")
 
+    def set_record_all(self, on: bool) -> None:
+        """Record every state in the forward pass (about K (32 N + 4 ncell)
+        bytes of device memory) so that the backward pass recomputes nothing;
+        off: checkpointed segments, recomputed backwards.  Same gradients."""
+        check(self.lib.psim_adj_set_record_all(self._adj, int(bool(on))),
+              "psim_adj_set_record_all")
+
     def set_cutoff_window(self, w: float) -> None:
         """w > 0 adds the pairs that cross the cutoff to dL/dcutoff, linearised
         over [cutoff - w, cutoff + w] (the force jumps there); 0 = pathwise."""

@@ -69,13 +69,18 @@ class AOFitProblem:
 
     def __init__(self, system: E.GpuSystem, nsteps: int, checkpoint_every: int = 20,
                  spec: GofRSpec | None = None, kT: float = 1.0,
-                 cutoff_window: float = 3e-3):
+                 cutoff_window: float = 3e-3, record_all: bool = True):
+        """record_all: keep every state of the trajectory from the forward
+        pass (about nsteps (32 N + 4 ncell) bytes of device memory, 0.26 GB at
+        N = 32768 and 200 steps) so that the backward pass recomputes nothing;
+        False: checkpoint_every-step segments, recomputed backwards."""
         self.system = system
         self.spec = spec or GofRSpec()
         self.kT = kT
         self.cutoff_window = float(cutoff_window)
         self.adjoint = E.Adjoint(system, nsteps, checkpoint_every)
         self.adjoint.set_cutoff_window(self.cutoff_window)
+        self.adjoint.set_record_all(record_all)
         n, L = system.n, system.geometry.box
         self.norm = n * (n / L ** 3) * self.spec.shell_volumes()
         self._theta = None
@@ -137,7 +142,7 @@ def gofr_loss(problem: AOFitProblem, theta: torch.Tensor, target: torch.Tensor):
 def c5_problem(n: int = 32768, phi: float = 0.15, nsteps: int = 200, every: int = 20,
                seed: int = 1234, noise_seed: int = 90210, dt: float = 1e-3,
                spec: GofRSpec | None = None, max_q: float = 0.2,
-               cutoff_window: float = 3e-3):
+               cutoff_window: float = 3e-3, record_all: bool = True):
     """The differentiable AO fit workload: N colloids at volume fraction phi,
     seeded RSA start, K Brownian steps with fixed in-kernel noise (the cell
     edge holds the cutoff 1 + max_q plus the cutoff window)."""
@@ -149,7 +154,8 @@ def c5_problem(n: int = 32768, phi: float = 0.15, nsteps: int = 200, every: int 
     pos = E.rsa_positions(n, geom.box, seed)
     vel = np.random.default_rng(seed + 1).standard_normal((n, 3))
     sysg.set_state(pos, vel=vel)
-    return AOFitProblem(sysg, nsteps, every, spec, cutoff_window=cutoff_window)
+    return AOFitProblem(sysg, nsteps, every, spec, cutoff_window=cutoff_window,
+                        record_all=record_all)
 
 
 def fit(problem: AOFitProblem, theta0, target_g: np.ndarray, iters: int = 20,

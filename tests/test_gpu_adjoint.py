@@ -134,6 +134,28 @@ def test_cutoff_window_beyond_the_cell_edge_is_refused():
             adj.set_cutoff_window(-1.0)
 
 
+def test_record_all_matches_checkpointing():
+    """Every state recorded by the forward pass (no recomputation) gives the
+    gradients of the checkpointed segments bit for bit, and switching back
+    works."""
+    case = "ao"
+    kw, n, L, S, K, every, seed, names = CASES[case]
+    w = np.random.default_rng(8).standard_normal((n, 3))
+    sysg, _, _ = make(case)
+    with sysg, ps.Adjoint(sysg, K, every) as adj:
+        res = []
+        for on in (False, True, False):
+            adj.set_record_all(on)
+            assert adj.forward() == 0
+            xk = sysg.get_state()["pos"]
+            res.append((adj.backward(w, with_gx0=True), xk))
+    (g0, x0), (g1, x1), (g2, x2) = res
+    assert np.array_equal(x0, x1) and np.array_equal(x0, x2)
+    for name in ps.engine.ADJOINT_PARAMS:
+        assert g1[name] == g0[name] and g2[name] == g0[name], name
+    assert np.array_equal(g1["gx0"], g0["gx0"]) and np.array_equal(g2["gx0"], g0["gx0"])
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     case = "ao"
     kw, n, L, S, K, every, seed, names = CASES[case]
