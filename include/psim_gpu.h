@@ -275,12 +275,19 @@ int psim_ens_last_error(psim_ens *ens, int *code, int *i, int *j);
  * index, partners named by their concatenated original index; as
  * psim_gpu_pair_stats). */
 int psim_ens_pair_stats(psim_ens *ens, int *count_by_orig, unsigned long long *hash_by_orig);
-/* measured choice of the pair kernel: *kernel 1 = thread per particle with
- * workgroup size *tile (64/128/256), 3 = bricks of cells per workgroup with
- * about *tile home particles per brick (96/192/384); the trial steps advance
- * the ensemble and the choice changes no result */
+/* measured choice of the pair kernel, made separately for each occupancy
+ * class of replicas (particles per cell < 0.25, < 0.5, < 1, >= 1; see
+ * psim_ens_class_info): kernel 1 = thread per particle with workgroup size
+ * tile (64/128/256), 3 = bricks of cells per workgroup with about tile home
+ * particles per brick (96/192/384).  *kernel and *tile report the class
+ * holding the most particles, *ms_per_step whole steps with every class's
+ * choice; those steps advance the ensemble, and no choice changes a result */
 int psim_ens_autotune(psim_ens *ens, int trial_steps, int *kernel, int *tile,
                       float *ms_per_step);
+/* occupancy class cls (0..PSIM_ENS_CLASSES-1): its number of replicas and its
+ * current pair kernel and tile (as psim_ens_autotune) */
+enum { PSIM_ENS_CLASSES = 4 };
+int psim_ens_class_info(psim_ens *ens, int cls, int *nrep, int *kernel, int *tile);
 
 /* ---- G&B integrator over a host's particle arrays -------------------------
  * The GPU side of the IIntegrator adaptor (include/psim/IIntegrator.h):

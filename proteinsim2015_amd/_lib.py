@@ -47,6 +47,7 @@ class Observables(C.Structure):
 
 
 DP_AO_WELL, DP_AO_CUTOFF, DP_LJ_WELL, DP_YUKAWA, DP_DEBYE, DP_COUNT = range(6)
+ENS_CLASSES = 4   # PSIM_ENS_CLASSES: occupancy classes of an ensemble's replicas
 
 
 class KernelTimes(C.Structure):
@@ -76,7 +77,7 @@ EXPORTS = (
     "psim_gpu_load", "psim_write_sys_config", "psim_gpu_kernel_info",
     "psim_gpu_pair_stats", "psim_ens_pair_stats", "psim_gpu_forces_noalloc",
     "psim_int_create", "psim_int_destroy", "psim_int_next", "psim_gpu_set_graph",
-    "psim_adj_set_cutoff_window", "psim_adj_set_record_all",
+    "psim_adj_set_cutoff_window", "psim_adj_set_record_all", "psim_ens_class_info",
 )
 
 _vp = C.c_void_p
@@ -131,6 +132,8 @@ _SIGS = {
     "psim_adj_set_initial": ([_vp], C.c_int),
     "psim_adj_set_cutoff_window": ([_vp, C.c_double], C.c_int),
     "psim_adj_set_record_all": ([_vp, C.c_int], C.c_int),
+    "psim_ens_class_info": ([_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                             C.POINTER(C.c_int)], C.c_int),
     "psim_adj_forward": ([_vp], C.c_int),
     "psim_adj_backward": ([_vp, _dp, _dp, _dp], C.c_int),
     "psim_init_reference": ([C.c_int, C.c_int, C.c_double, C.c_uint, C.c_int, C.c_double, _dp,

@@ -58,6 +58,7 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
     with E.Ensemble(reps, integ) as ens:
         ens.set_state(np.concatenate(pos), np.concatenate(vel))
         kernel, tile, _ = ens.autotune(3)
+        classes = ens.class_choices()
         if not eager:
             ens.set_graph(graph_steps)
         ens.step(warmup)
@@ -121,6 +122,9 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
         "speedup_vs_graph_baseline": batched / single_graph if single_graph else None,
         "autotuned_kernel": kernel,
         "autotuned_tile": tile,
+        "occupancy_classes": [{"particles_per_cell": lim, "replicas": n, "kernel": k, "tile": t}
+                              for lim, (n, k, t) in zip(("<0.25", "<0.5", "<1", ">=1"),
+                                                        classes)],
         "graph_steps": None if eager else graph_steps,
     }
 

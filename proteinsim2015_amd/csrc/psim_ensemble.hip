@@ -7,9 +7,9 @@
 //   cells: replica r owns global cells [c0_r, c0_r + S_r^3), so one counting
 //     sort over the global cell keys rebuilds every replica's table at once
 //     and the replicas stay contiguous in slot order.
-// Each step is one launch per kernel stage for all replicas (pair forces +
-// G&B, scan, scatter, reorder, control), so a step of 512 replicas costs the
-// same number of launches as one system, and the sequence can be captured
+// Each step is one launch per kernel stage for all replicas (pair forces,
+// one per occupancy class, then G&B, scan, scatter, reorder), so a step of
+// 512 replicas costs a handful of launches, and the sequence can be captured
 // once in a hipGraph and replayed.  The step counter, time and velocity-step
 // flags live in device memory (StepCtl) so replays advance them.
 //
@@ -17,8 +17,11 @@
 // same bits: the thread-per-particle gather (k_ens_step, the default) and
 // bricks of cells per workgroup (k_ens_step_brick: the launch's workgroups
 // are the bricks of every replica, each replica cut by its own density).
-// The autotune times both families over their tiles (kEnsBlocks workgroup
-// sizes, kEnsTargets brick sizes) and keeps the fastest.
+// The replicas fall into occupancy classes (particles per cell), and the
+// autotune times both families over their tiles (kEnsBlocks workgroup sizes,
+// kEnsTargets brick sizes) for each class's replicas and keeps the fastest
+// for that class: the pair stage is one launch per class, the integrator one
+// launch for all.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,9 +59,11 @@ __device__ __forceinline__ int replica_of(const int *__restrict__ p0, int nrep, 
     return lo;
 }
 
-// Thread-per-particle pair forces: workgroup `it` of the launch covers slots
-// p0[r] + (it - blk0[r]) BLOCK .. of replica r (blk0 = prefix of the
-// replicas' workgroup counts), so a workgroup never spans two replicas and
+// Thread-per-particle pair forces for the replicas rmap[0..nrep): workgroup
+// `it` of the launch covers slots p0[r] + (it - blk0[l]) BLOCK .. of replica
+// r = rmap[l] (blk0 = prefix of those replicas' workgroup counts; the
+// replicas of one occupancy class, see EnsClass), so a workgroup never spans
+// two replicas and
 // the replica's geometry and constants are the same for every lane (scalar
 // loads, no per-lane replica search).  The forces go to nfrc, where the
 // integrator (k_ens_integ, the same launch shape) reads them: as two
@@ -67,14 +72,16 @@ __device__ __forceinline__ int replica_of(const int *__restrict__ p0, int nrep, 
 template <int MASK, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_ens_step(const RepGeo *__restrict__ reps,
                                                     const int *__restrict__ p0,
-                                                    const int *__restrict__ blk0, int nrep,
+                                                    const int *__restrict__ blk0,
+                                                    const int *__restrict__ rmap, int nrep,
                                                     int nblk, int per_xcd, int ntot, StepArgs a)
 {
     __shared__ unsigned hits[kHitCap * BLOCK];
     const int it = xcd_cell(blockIdx.x, per_xcd);
     if (it >= nblk) return;
-    const int r = replica_of(blk0, nrep, it);
-    const int s = p0[r] + (it - blk0[r]) * BLOCK + (int)threadIdx.x;
+    const int rl = replica_of(blk0, nrep, it);   // index in the launch's replica list
+    const int r = rmap[rl];
+    const int s = p0[r] + (it - blk0[rl]) * BLOCK + (int)threadIdx.x;
     if (s >= p0[r + 1]) return;
     // references into the replica table: local copies passed on by
     // reference would live in scratch memory
@@ -115,26 +122,31 @@ __global__ __launch_bounds__(BLOCK) void k_ens_integ(const RepGeo *__restrict__ 
     integrate_one<false>(reps[r].g, gb, a, s, a.nfrc[s], a.nfrc[N + s], a.nfrc[2 * N + s]);
 }
 
-// Brick kernel: workgroup `it` of the launch is brick it - item0[r] of
-// replica r (item0 = prefix of the replicas' brick counts).
+// Brick kernel: workgroup `it` of the launch is brick it - item0[l] of
+// replica r = rmap[l] (item0 = prefix of the listed replicas' brick counts);
+// the forces go to nfrc for k_ens_integ.
 template <int MASK>
 __global__ __launch_bounds__(kBrickThreads) void k_ens_step_brick(
-    const RepGeo *__restrict__ reps, const int *__restrict__ item0, int nrep, int nitems,
-    int per_xcd, GB gb, StepArgs a)
+    const RepGeo *__restrict__ reps, const int *__restrict__ item0,
+    const int *__restrict__ rmap, int nrep, int nitems, int per_xcd, int ntot, StepArgs a)
 {
     __shared__ BrickShared sh;
     const int it = xcd_cell(blockIdx.x, per_xcd);
     if (it >= nitems) return;
-    const int r = replica_of(item0, nrep, it);
+    const int rl = replica_of(item0, nrep, it);
+    const int r = rmap[rl];
     // references into the replica table: local copies passed on by
     // reference would live in scratch memory
     const Geo &g = reps[r].g;
     const Pot &p = reps[r].p;
     const double radius = reps[r].radius;
     const BrickGeo &bg = reps[r].bg;
-    const int b = it - item0[r];
+    const int b = it - item0[rl];
+    const int N = ntot;
     auto epi = [&](int s, double fx, double fy, double fz) {
-        integrate_one<true>(g, gb, a, s, fx, fy, fz);
+        a.nfrc[s] = fx;
+        a.nfrc[N + s] = fy;
+        a.nfrc[2 * N + s] = fz;
     };
     if (g.exact_mi)
         brick_forces<MASK, true, 1>(g, p, radius, a.pos, a.cstart, a.id, a.err, bg, b, sh, epi);
@@ -148,7 +160,8 @@ __global__ __launch_bounds__(kBrickThreads) void k_ens_step_brick(
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_ens_stats(const RepGeo *__restrict__ reps,
                                                      const int *__restrict__ p0,
-                                                     const int *__restrict__ blk0, int nrep,
+                                                     const int *__restrict__ blk0,
+                                                     const int *__restrict__ rmap, int nrep,
                                                      int nblk, int per_xcd, int ntot,
                                                      const double *pos, const int *id,
                                                      const int *cstart, NbrStatsOut so)
@@ -156,8 +169,9 @@ __global__ __launch_bounds__(BLOCK) void k_ens_stats(const RepGeo *__restrict__ 
     __shared__ unsigned hits[kHitCap * BLOCK];
     const int it = xcd_cell(blockIdx.x, per_xcd);
     if (it >= nblk) return;
-    const int r = replica_of(blk0, nrep, it);
-    const int s = p0[r] + (it - blk0[r]) * BLOCK + (int)threadIdx.x;
+    const int rl = replica_of(blk0, nrep, it);
+    const int r = rmap[rl];
+    const int s = p0[r] + (it - blk0[rl]) * BLOCK + (int)threadIdx.x;
     if (s >= p0[r + 1]) return;
     const Geo &g = reps[r].g;
     const Pot &p = reps[r].p;
@@ -180,18 +194,20 @@ __global__ __launch_bounds__(BLOCK) void k_ens_stats(const RepGeo *__restrict__ 
 }
 
 __global__ __launch_bounds__(kBrickThreads) void k_ens_stats_brick(
-    const RepGeo *__restrict__ reps, const int *__restrict__ item0, int nrep, int nitems,
-    int per_xcd, const double *pos, const int *id, const int *cstart, NbrStatsOut so)
+    const RepGeo *__restrict__ reps, const int *__restrict__ item0,
+    const int *__restrict__ rmap, int nrep, int nitems, int per_xcd, const double *pos,
+    const int *id, const int *cstart, NbrStatsOut so)
 {
     __shared__ BrickShared sh;
     const int it = xcd_cell(blockIdx.x, per_xcd);
     if (it >= nitems) return;
-    const int r = replica_of(item0, nrep, it);
+    const int rl = replica_of(item0, nrep, it);
+    const int r = rmap[rl];
     const Geo &g = reps[r].g;
     const Pot &p = reps[r].p;
     const double radius = reps[r].radius;
     const BrickGeo &bg = reps[r].bg;
-    const int b = it - item0[r];
+    const int b = it - item0[rl];
     auto epi = [](int, double, double, double) {};
     if (g.exact_mi)
         brick_forces<F_CAL, true, 1, NbrStats>(g, p, radius, pos, cstart, id, nullptr, bg, b, sh,
@@ -225,6 +241,22 @@ constexpr int kEnsBlocks[3] = {64, 128, 256};
 // home particles per brick tried by the autotune (one lane per particle)
 constexpr int kEnsTargets[3] = {96, 192, 384};
 
+// Occupancy classes (particles per cell of a replica): [0, 0.25),
+// [0.25, 0.5), [0.5, 1), [1, inf).  Each class's replicas get their own pair
+// kernel and tile, which the autotune measures class by class.
+constexpr int kEnsClasses = 4;
+inline int occ_class(double occ) { return occ < 0.25 ? 0 : occ < 0.5 ? 1 : occ < 1.0 ? 2 : 3; }
+
+struct EnsClass {
+    std::vector<int> reps;     // replica ids, ascending
+    int *rmap = nullptr;       // the same list on the device
+    bool brick = false;        // brick kernel (else thread per particle)
+    int block = 256;           // thread-per-particle workgroup size
+    int target = 192;          // brick kernel: home particles per brick
+    int nblk = 0, nitems = 0;  // workgroups / bricks over the class's replicas
+    int *blk0 = nullptr, *item0 = nullptr;   // their prefixes over the list
+};
+
 }  // namespace
 
 struct psim_ens {
@@ -232,13 +264,12 @@ struct psim_ens {
     GB gb;
     int nrep = 0, ntot = 0, ncell = 0, nscan = 0;
     int mask = 0;
-    bool brick = false;  // brick kernel (else thread per particle)
-    int block = 256;     // thread-per-particle workgroup size
-    int target = 192;    // brick kernel: home particles per brick
-    int nitems = 0;      // bricks over all replicas
-    int *item0 = nullptr;
-    int nblk = 0;        // thread-per-particle workgroups over all replicas
-    int *blk0 = nullptr; // prefix of the replicas' workgroup counts
+    bool brick = false;  // initial pair kernel of every class: bricks
+    int block = 256;     // initial workgroup size / brick target
+    int target = 192;
+    EnsClass cls[kEnsClasses];
+    int integ_nblk = 0;        // integrator: 256-lane workgroups over all replicas
+    int *integ_blk0 = nullptr; // prefix of the replicas' workgroup counts
     std::vector<RepGeo> h_reps;
     std::vector<int> h_occ_n;
     hipStream_t stream = nullptr;
@@ -309,29 +340,39 @@ StepArgs ens_args(psim_ens *e)
 }
 
 template <int MASK, int BLOCK>
-void launch_ens_pair(psim_ens *e, const StepArgs &a)
+void launch_ens_pair(psim_ens *e, const EnsClass &cl, const StepArgs &a)
 {
-    const int per = (e->nblk + 7) / 8;
+    const int per = (cl.nblk + 7) / 8;
     k_ens_step<MASK, BLOCK><<<8 * per, BLOCK, 0, e->stream>>>(
-        e->reps, e->p0, e->blk0, e->nrep, e->nblk, per, e->ntot, a);
-    k_ens_integ<BLOCK><<<8 * per, BLOCK, 0, e->stream>>>(
-        e->reps, e->p0, e->blk0, e->nrep, e->nblk, per, e->ntot, e->gb, a);
+        e->reps, e->p0, cl.blk0, cl.rmap, (int)cl.reps.size(), cl.nblk, per, e->ntot, a);
 }
 
+// pair forces of one occupancy class's replicas into nfrc
+template <int MASK>
+void launch_ens_pair_class(psim_ens *e, const EnsClass &cl, const StepArgs &a)
+{
+    if (cl.reps.empty()) return;
+    if (cl.brick) {
+        const int per = (cl.nitems + 7) / 8;
+        k_ens_step_brick<MASK><<<8 * per, kBrickThreads, 0, e->stream>>>(
+            e->reps, cl.item0, cl.rmap, (int)cl.reps.size(), cl.nitems, per, e->ntot, a);
+        return;
+    }
+    switch (cl.block) {
+    case 64: launch_ens_pair<MASK, 64>(e, cl, a); break;
+    case 128: launch_ens_pair<MASK, 128>(e, cl, a); break;
+    default: launch_ens_pair<MASK, 256>(e, cl, a); break;
+    }
+}
+
+// pair forces of every class, then the G&B step of every particle
 template <int MASK>
 void launch_ens_pair_block(psim_ens *e, const StepArgs &a)
 {
-    if (e->brick) {
-        const int per = (e->nitems + 7) / 8;
-        k_ens_step_brick<MASK><<<8 * per, kBrickThreads, 0, e->stream>>>(
-            e->reps, e->item0, e->nrep, e->nitems, per, e->gb, a);
-        return;
-    }
-    switch (e->block) {
-    case 64: launch_ens_pair<MASK, 64>(e, a); break;
-    case 128: launch_ens_pair<MASK, 128>(e, a); break;
-    default: launch_ens_pair<MASK, 256>(e, a); break;
-    }
+    for (const EnsClass &cl : e->cls) launch_ens_pair_class<MASK>(e, cl, a);
+    const int per = (e->integ_nblk + 7) / 8;
+    k_ens_integ<256><<<8 * per, 256, 0, e->stream>>>(e->reps, e->p0, e->integ_blk0, e->nrep,
+                                                     e->integ_nblk, per, e->ntot, e->gb, a);
 }
 
 void launch_build_ens(psim_ens *e, StepCtl *advance)
@@ -376,18 +417,54 @@ int settle_ens(psim_ens *e)
 
 // Workgroups of `block` lanes per replica for the thread-per-particle
 // kernel, and their prefix.
-int set_blocks(psim_ens *e, int block)
+// Thread-per-particle workgroups of `block` lanes for the replicas of class c.
+int set_blocks(psim_ens *e, int c, int block)
+{
+    EnsClass &cl = e->cls[c];
+    const int m = (int)cl.reps.size();
+    std::vector<int> b0(m + 1);
+    int nb = 0;
+    for (int l = 0; l < m; ++l) {
+        b0[l] = nb;
+        nb += (e->h_occ_n[cl.reps[l]] + block - 1) / block;
+    }
+    b0[m] = nb;
+    if (m) ECK(hipMemcpy(cl.blk0, b0.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice));
+    cl.brick = false;
+    cl.block = block;
+    cl.nblk = nb;
+    return 0;
+}
+
+// The integrator's 256-lane workgroups over all replicas.
+int set_integ_blocks(psim_ens *e)
 {
     std::vector<int> b0(e->nrep + 1);
     int nb = 0;
     for (int r = 0; r < e->nrep; ++r) {
         b0[r] = nb;
-        nb += (e->h_occ_n[r] + block - 1) / block;
+        nb += (e->h_occ_n[r] + 255) / 256;
     }
     b0[e->nrep] = nb;
-    ECK(hipMemcpy(e->blk0, b0.data(), sizeof(int) * (e->nrep + 1), hipMemcpyHostToDevice));
-    e->block = block;
-    e->nblk = nb;
+    ECK(hipMemcpy(e->integ_blk0, b0.data(), sizeof(int) * (e->nrep + 1),
+                  hipMemcpyHostToDevice));
+    e->integ_nblk = nb;
+    return 0;
+}
+
+// The pair kernel of one class (forces into nfrc only).
+int launch_class(psim_ens *e, const EnsClass &cl, const StepArgs &a)
+{
+    switch (e->mask) {
+    case 1: launch_ens_pair_class<1>(e, cl, a); break;
+    case 2: launch_ens_pair_class<2>(e, cl, a); break;
+    case 3: launch_ens_pair_class<3>(e, cl, a); break;
+    case 4: launch_ens_pair_class<4>(e, cl, a); break;
+    case 5: launch_ens_pair_class<5>(e, cl, a); break;
+    case 6: launch_ens_pair_class<6>(e, cl, a); break;
+    case 7: launch_ens_pair_class<7>(e, cl, a); break;
+    default: return PSIM_E_ARG;
+    }
     return 0;
 }
 
@@ -422,25 +499,29 @@ void drop_graph(psim_ens *e)
     }
 }
 
-// Cut every replica into bricks of about `target` home particles and upload
-// the replica table and the brick prefix.
-int set_bricks(psim_ens *e, int target)
+// Cut every replica of class c into bricks of about `target` home particles
+// and upload the replica table and the class's brick prefix.
+int set_bricks(psim_ens *e, int c, int target)
 {
-    std::vector<int> it0(e->nrep + 1);
+    EnsClass &cl = e->cls[c];
+    const int m = (int)cl.reps.size();
+    std::vector<int> it0(m + 1);
     int nitems = 0;
-    for (int r = 0; r < e->nrep; ++r) {
+    for (int l = 0; l < m; ++l) {
+        const int r = cl.reps[l];
         RepGeo &rg = e->h_reps[r];
         const double occ = (double)e->h_occ_n[r] / rg.g.ncell;
         rg.bg = brick_shape(rg.g.S, occ, (double)target);
-        it0[r] = nitems;
+        it0[l] = nitems;
         nitems += rg.bg.nbricks;
     }
-    it0[e->nrep] = nitems;
+    it0[m] = nitems;
     ECK(hipMemcpy(e->reps, e->h_reps.data(), sizeof(RepGeo) * e->nrep,
                   hipMemcpyHostToDevice));
-    ECK(hipMemcpy(e->item0, it0.data(), sizeof(int) * (e->nrep + 1), hipMemcpyHostToDevice));
-    e->target = target;
-    e->nitems = nitems;
+    if (m) ECK(hipMemcpy(cl.item0, it0.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice));
+    cl.brick = true;
+    cl.target = target;
+    cl.nitems = nitems;
     return 0;
 }
 
@@ -572,13 +653,28 @@ int psim_ens_create(const psim_params *common, const psim_replica *reps, int nre
     if (!rc) rc = ealloc(&e->ctl, 1);
     if (!rc) rc = ealloc(&e->reps, (size_t)nrep);
     if (!rc) rc = ealloc(&e->p0, (size_t)nrep + 1);
-    if (!rc) rc = ealloc(&e->item0, (size_t)nrep + 1);
-    if (!rc) rc = ealloc(&e->blk0, (size_t)nrep + 1);
+    if (!rc) rc = ealloc(&e->integ_blk0, (size_t)nrep + 1);
     e->h_reps = h;
     e->h_occ_n.resize(nrep);
-    for (int r = 0; r < nrep; ++r) e->h_occ_n[r] = reps[r].n_particles;
-    if (!rc) rc = set_bricks(e, e->target);
-    if (!rc) rc = set_blocks(e, e->block);
+    for (int r = 0; r < nrep; ++r) {
+        e->h_occ_n[r] = reps[r].n_particles;
+        e->cls[occ_class((double)reps[r].n_particles / h[r].g.ncell)].reps.push_back(r);
+    }
+    for (int c = 0; c < kEnsClasses && !rc; ++c) {
+        EnsClass &cl = e->cls[c];
+        if (cl.reps.empty()) continue;
+        const size_t m = cl.reps.size();
+        rc = ealloc(&cl.rmap, m);
+        if (!rc) rc = ealloc(&cl.blk0, m + 1);
+        if (!rc) rc = ealloc(&cl.item0, m + 1);
+        if (!rc && hipMemcpy(cl.rmap, cl.reps.data(), sizeof(int) * m, hipMemcpyHostToDevice) !=
+                       hipSuccess)
+            rc = PSIM_E_HIP;
+        if (!rc) rc = set_blocks(e, c, e->block);
+        if (!rc) rc = set_bricks(e, c, e->target);   // the replica table with its bricks
+        if (!rc && !e->brick) rc = set_blocks(e, c, e->block);
+    }
+    if (!rc) rc = set_integ_blocks(e);
     if (!rc && hipMemcpy(e->p0, e->h_p0.data(), sizeof(int) * (nrep + 1), hipMemcpyHostToDevice) != hipSuccess)
         rc = PSIM_E_HIP;
     if (!rc && hipMemset(e->count, 0, sizeof(int) * e->ncell) != hipSuccess) rc = PSIM_E_HIP;
@@ -598,10 +694,15 @@ void psim_ens_destroy(psim_ens *e)
     drop_graph(e);
     void *bufs[] = {e->pos, e->pos0, e->frc, e->mem, e->npos, e->npos0, e->nfrc, e->nmem,
                     e->vel, e->id, e->nid, e->key, e->arank, e->perm, e->count, e->cstart,
-                    e->bsum, e->err, e->ctl, e->reps, e->p0, e->item0, e->st_count,
-                    e->st_hash, e->blk0, e->prev};
+                    e->bsum, e->err, e->ctl, e->reps, e->p0, e->integ_blk0, e->st_count,
+                    e->st_hash, e->prev};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    for (EnsClass &cl : e->cls) {
+        void *cb[] = {cl.rmap, cl.blk0, cl.item0};
+        for (void *b : cb)
+            if (b) (void)hipFree(b);
+    }
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -750,16 +851,21 @@ int psim_ens_pair_stats(psim_ens *e, int *count_by_orig, unsigned long long *has
         if (rc) return rc;
     }
     const NbrStatsOut so{e->st_count, e->st_hash};
-    if (e->brick) {
-        const int per = (e->nitems + 7) / 8;
-        k_ens_stats_brick<<<8 * per, kBrickThreads, 0, e->stream>>>(
-            e->reps, e->item0, e->nrep, e->nitems, per, e->pos, e->id, e->cstart, so);
-    } else {
-        const int per = (e->nblk + 7) / 8;
+    for (const EnsClass &cl : e->cls) {
+        if (cl.reps.empty()) continue;
+        const int m = (int)cl.reps.size();
+        if (cl.brick) {
+            const int per = (cl.nitems + 7) / 8;
+            k_ens_stats_brick<<<8 * per, kBrickThreads, 0, e->stream>>>(
+                e->reps, cl.item0, cl.rmap, m, cl.nitems, per, e->pos, e->id, e->cstart, so);
+            continue;
+        }
+        const int per = (cl.nblk + 7) / 8;
 #define PSIM_ENS_STATS(BL)                                                                 \
-    k_ens_stats<BL><<<8 * per, BL, 0, e->stream>>>(e->reps, e->p0, e->blk0, e->nrep, e->nblk, \
-                                                   per, n, e->pos, e->id, e->cstart, so)
-        switch (e->block) {
+    k_ens_stats<BL><<<8 * per, BL, 0, e->stream>>>(e->reps, e->p0, cl.blk0, cl.rmap, m,    \
+                                                   cl.nblk, per, n, e->pos, e->id,         \
+                                                   e->cstart, so)
+        switch (cl.block) {
         case 64: PSIM_ENS_STATS(64); break;
         case 128: PSIM_ENS_STATS(128); break;
         default: PSIM_ENS_STATS(256); break;
@@ -775,10 +881,14 @@ int psim_ens_pair_stats(psim_ens *e, int *count_by_orig, unsigned long long *has
     return 0;
 }
 
-// Time every candidate (three workgroup sizes of the thread-per-particle
-// kernel, three brick sizes of the brick kernel) for trial_steps steps (after
-// one warm-up step each) and keep the fastest.  The steps taken while tuning
-// advance the ensemble like any other steps; the choice changes no result.
+// For each occupancy class: time every candidate pair kernel (three workgroup
+// sizes of the thread-per-particle kernel, three brick sizes of the brick
+// kernel) on that class's replicas alone, trial_steps launches after one
+// warm-up (the pair kernels only write forces, so they repeat on one state),
+// and keep the fastest.  Then trial_steps whole steps with the choices give
+// *best_ms; *kernel and *tile report the choice of the class holding the most
+// particles (psim_ens_class_info gives every class's).  The whole steps
+// advance the ensemble like any other steps; no choice changes a result.
 int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_kernel, int *chosen_tile,
                       float *best_ms)
 {
@@ -786,48 +896,70 @@ int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_kernel, int *cho
     hipEvent_t t0, t1;
     ECK(hipEventCreate(&t0));
     ECK(hipEventCreate(&t1));
-    float best = 0.f;
-    bool best_brick = e->brick;
-    int best_block = e->brick ? e->target : e->block;
-    for (int ci = 0; ci < 6; ++ci) {
-        const bool brick = ci >= 3;
-        const int b = brick ? kEnsTargets[ci - 3] : kEnsBlocks[ci];
-        int rc = 0;
-        e->brick = brick;
-        if (brick) {
-            if ((rc = set_bricks(e, b))) return rc;
-        } else {
-            if ((rc = set_blocks(e, b))) return rc;
+    int rc = 0;
+    int big = -1;
+    long long big_n = -1;
+    const StepArgs a = ens_args(e);
+    for (int c = 0; c < kEnsClasses && !rc; ++c) {
+        EnsClass &cl = e->cls[c];
+        if (cl.reps.empty()) continue;
+        float best = 0.f;
+        bool best_brick = cl.brick;
+        int best_tile = cl.brick ? cl.target : cl.block;
+        for (int ci = 0; ci < 6 && !rc; ++ci) {
+            const bool brick = ci >= 3;
+            const int b = brick ? kEnsTargets[ci - 3] : kEnsBlocks[ci];
+            rc = brick ? set_bricks(e, c, b) : set_blocks(e, c, b);
+            if (!rc) rc = launch_class(e, cl, a);
+            if (rc) break;
+            ECK(hipEventRecord(t0, e->stream));
+            for (int k = 0; k < trial_steps && !rc; ++k) rc = launch_class(e, cl, a);
+            ECK(hipEventRecord(t1, e->stream));
+            ECK(hipEventSynchronize(t1));
+            ECK(hipGetLastError());
+            float ms = 0.f;
+            ECK(hipEventElapsedTime(&ms, t0, t1));
+            if (best == 0.f || ms < best) {
+                best = ms;
+                best_tile = b;
+                best_brick = brick;
+            }
         }
-        rc = enqueue_step(e);
-        if (rc) return rc;
+        if (!rc) rc = best_brick ? set_bricks(e, c, best_tile) : set_blocks(e, c, best_tile);
+        long long nc = 0;
+        for (int r : cl.reps) nc += e->h_occ_n[r];
+        if (nc > big_n) {
+            big_n = nc;
+            big = c;
+        }
+    }
+    drop_graph(e);   // recaptured with the chosen kernels on the next step
+    float ms = 0.f;
+    if (!rc) rc = enqueue_step(e);
+    if (!rc) {
         ECK(hipEventRecord(t0, e->stream));
-        for (int k = 0; k < trial_steps; ++k)
-            if ((rc = enqueue_step(e))) return rc;
+        for (int k = 0; k < trial_steps && !rc; ++k) rc = enqueue_step(e);
         ECK(hipEventRecord(t1, e->stream));
         ECK(hipEventSynchronize(t1));
-        float ms = 0.f;
         ECK(hipEventElapsedTime(&ms, t0, t1));
-        if (best == 0.f || ms < best) {
-            best = ms;
-            best_block = b;
-            best_brick = brick;
-        }
     }
-    e->brick = best_brick;
-    if (e->brick) {
-        int rc = set_bricks(e, best_block);
-        if (rc) return rc;
-    } else {
-        int rc = set_blocks(e, best_block);
-        if (rc) return rc;
-    }
-    drop_graph(e);   // recaptured with the chosen kernel on the next step
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    if (chosen_kernel) *chosen_kernel = best_brick ? 3 : 1;
-    if (chosen_tile) *chosen_tile = best_block;
-    if (best_ms) *best_ms = best / trial_steps;
+    if (rc) return rc;
+    const EnsClass &cb = e->cls[big];
+    if (chosen_kernel) *chosen_kernel = cb.brick ? 3 : 1;
+    if (chosen_tile) *chosen_tile = cb.brick ? cb.target : cb.block;
+    if (best_ms) *best_ms = ms / trial_steps;
+    return 0;
+}
+
+int psim_ens_class_info(psim_ens *e, int cls, int *nrep, int *kernel, int *tile)
+{
+    if (!e || cls < 0 || cls >= kEnsClasses) return PSIM_E_ARG;
+    const EnsClass &cl = e->cls[cls];
+    if (nrep) *nrep = (int)cl.reps.size();
+    if (kernel) *kernel = cl.brick ? 3 : 1;
+    if (tile) *tile = cl.brick ? cl.target : cl.block;
     return 0;
 }
 

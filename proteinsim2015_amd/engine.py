@@ -715,12 +715,25 @@ class Ensemble:
         return cnt, hs
 
     def autotune(self, trial_steps: int = 5):
-        """Time the pair-kernel candidates (thread per particle with workgroup
-        sizes 64/128/256, bricks of about 96/192/384 home particles) and keep
-        the fastest; returns (kernel, tile, ms per step), kernel "sparse" or
-        "brick".  The trial steps advance the ensemble; results do not depend
-        on the choice."""
+        """For each occupancy class of replicas (particles per cell below 0.25,
+        0.5, 1, and above), time the pair-kernel candidates (thread per
+        particle with workgroup sizes 64/128/256, bricks of about 96/192/384
+        home particles) on that class and keep the fastest; returns (kernel,
+        tile, ms per step) of the class with the most particles, kernel
+        "sparse" or "brick" (``class_choices`` lists every class).  The timed
+        whole steps advance the ensemble; results do not depend on the
+        choice."""
         k, t, ms = C.c_int(), C.c_int(), C.c_float()
         check(self.lib.psim_ens_autotune(self._ctx, int(trial_steps), C.byref(k), C.byref(t),
                                          C.byref(ms)), "psim_ens_autotune")
         return ("brick" if k.value == 3 else "sparse"), t.value, ms.value
+
+    def class_choices(self):
+        """[(replicas, kernel, tile)] for each occupancy class, in order."""
+        out = []
+        for c in range(_lib.ENS_CLASSES):
+            n, k, t = C.c_int(), C.c_int(), C.c_int()
+            check(self.lib.psim_ens_class_info(self._ctx, c, C.byref(n), C.byref(k), C.byref(t)),
+                  "psim_ens_class_info")
+            out.append((n.value, "brick" if k.value == 3 else "sparse", t.value))
+        return out
