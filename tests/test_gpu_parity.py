@@ -278,3 +278,52 @@ def test_full_size_c3_properties(oracle, path):
     err = force_rel_err(f, fo, denom)
     assert err.max() <= FORCE_TOL, f"max {err.max():.3g} at {int(err.argmax())}"
     assert np.abs(f - fo).max() <= FORCE_TOL * np.abs(fo).max()
+
+
+def test_c1_thousand_steps_follow_the_oracle(oracle):
+    """C1 at its own size and length: N = 4096, Calibration + LJ 12-6 (eps 1,
+    cutoff 2.5), phi 0.20 (exact L, 8^3 cells), dt 1e-3, 1000 steps with
+    in-kernel Philox noise, against the oracle's trajectory with the same
+    noise.  The two differ by rounding only, which the dynamics amplifies
+    slowly: positions within 1e-8 after 100, 500 and 1000 steps, and the
+    neighbour sets equal at the end."""
+    from oracle.oracle import PO_CAL, PO_LJ, ForceParams
+    n = 4096
+    geom = ps.SystemGeometry.exact(n, 0.20, 2.5)
+    mods = [ps.Calibration(cutoff=2.5),
+            ps.LennardJones(kT=1.0, well_depth=1.0, yukawa_strength=0.0, lj_num=6, cutoff=2.5,
+                            debye_length=0.5)]
+    integ = ps.BrownianIntegrator(kT=1.0, mass=1.0, gamma=1.0, dt=1e-3, vel_freq=1000,
+                                  seed=90210)
+    pos = ps.rsa_positions(n, geom.box, 1234)
+    vel = np.random.default_rng(1235).standard_normal((n, 3))
+    marks = (100, 500, 1000)
+    print("c1: 1000 steps", flush=True)
+    gpu = []
+    with ps.GpuSystem(geom, mods, integ) as sysg:
+        sysg.set_state(pos, vel=vel)
+        done = 0
+        for m in marks:
+            assert sysg.step(m - done) == 0
+            done = m
+            gpu.append(sysg.get_state()["pos"])
+        cnt, hs = sysg.pair_stats()
+    L, S, cell = geom.box, geom.cells_per_dim, geom.cell_size
+    fp = ForceParams.make(PO_CAL | PO_LJ, 2.5, kT=1.0, lj_well_depth=1.0, yukawa_strength=0.0,
+                          debye_length=0.5, lj_num=6)
+    c = oracle.gb_coeffs(1.0, 1.0, 1.0, 1e-3)
+    state = dict(pos=pos.copy(), pos0=np.where(np.abs(pos) > L / 2, np.where(pos < 0, -L, L), 0.0),
+                 vel=vel.copy(), frc=np.zeros((n, 3)), frc0=np.zeros((n, 3)),
+                 memx=np.zeros((n, 3)))
+    done = 0
+    for m, gp in zip(marks, gpu):
+        print(f"c1: oracle to step {m}", flush=True)
+        st, _ = oracle.run(fp, c, state, m - done, L, S, cell, vel_freq=1000, seed=90210)
+        assert st == 0
+        done = m
+        d = gp - state["pos"]
+        err = np.abs(d - L * np.round(d / L)).max()   # a wrap on one side only
+        print(f"c1: step {m}: max |dx| {err:.3g}", flush=True)
+        assert err <= 1e-8, (m, err)
+    ocnt, ohs = oracle.pair_stats(fp, state["pos"], S, cell, L)
+    assert np.array_equal(cnt, ocnt) and np.array_equal(hs, ohs)
