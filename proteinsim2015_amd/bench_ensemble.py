@@ -57,7 +57,7 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
     # ---- batched: one launch per kernel stage for all replicas, graph replay
     with E.Ensemble(reps, integ) as ens:
         ens.set_state(np.concatenate(pos), np.concatenate(vel))
-        kernel, tile, _ = ens.autotune(3)
+        kernel, tile, _ = ens.autotune(10)
         classes = ens.class_choices()
         if not eager:
             ens.set_graph(graph_steps)
