@@ -279,15 +279,20 @@ int psim_ens_pair_stats(psim_ens *ens, int *count_by_orig, unsigned long long *h
  * class of replicas (particles per cell < 0.25, < 0.5, < 1, >= 1; see
  * psim_ens_class_info): kernel 1 = thread per particle with workgroup size
  * tile (64/128/256), 3 = bricks of cells per workgroup with about tile home
- * particles per brick (96/192/384).  *kernel and *tile report the class
- * holding the most particles, *ms_per_step whole steps with every class's
- * choice; those steps advance the ensemble, and no choice changes a result */
+ * particles per brick (96/192/384); the same is timed for all replicas in
+ * one launch, and whole steps decide between one launch per class and one
+ * for all (psim_ens_merged).  *kernel and *tile report the class holding the
+ * most particles, *ms_per_step the chosen configuration's whole steps; those
+ * steps advance the ensemble, and no choice changes a result */
 int psim_ens_autotune(psim_ens *ens, int trial_steps, int *kernel, int *tile,
                       float *ms_per_step);
-/* occupancy class cls (0..PSIM_ENS_CLASSES-1): its number of replicas and its
- * current pair kernel and tile (as psim_ens_autotune) */
+/* occupancy class cls (0..PSIM_ENS_CLASSES-1): its number of replicas and the
+ * pair kernel and tile that run them (as psim_ens_autotune) */
 enum { PSIM_ENS_CLASSES = 4 };
 int psim_ens_class_info(psim_ens *ens, int cls, int *nrep, int *kernel, int *tile);
+/* 1 when the autotune found one pair launch for all replicas faster than one
+ * per class (every class then reports that launch's kernel and tile), else 0 */
+int psim_ens_merged(psim_ens *ens);
 
 /* ---- G&B integrator over a host's particle arrays -------------------------
  * The GPU side of the IIntegrator adaptor (include/psim/IIntegrator.h):

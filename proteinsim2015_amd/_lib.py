@@ -78,6 +78,7 @@ EXPORTS = (
     "psim_gpu_pair_stats", "psim_ens_pair_stats", "psim_gpu_forces_noalloc",
     "psim_int_create", "psim_int_destroy", "psim_int_next", "psim_gpu_set_graph",
     "psim_adj_set_cutoff_window", "psim_adj_set_record_all", "psim_ens_class_info",
+    "psim_ens_merged",
 )
 
 _vp = C.c_void_p
@@ -134,6 +135,7 @@ _SIGS = {
     "psim_adj_set_record_all": ([_vp, C.c_int], C.c_int),
     "psim_ens_class_info": ([_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                              C.POINTER(C.c_int)], C.c_int),
+    "psim_ens_merged": ([_vp], C.c_int),
     "psim_adj_forward": ([_vp], C.c_int),
     "psim_adj_backward": ([_vp, _dp, _dp, _dp], C.c_int),
     "psim_init_reference": ([C.c_int, C.c_int, C.c_double, C.c_uint, C.c_int, C.c_double, _dp,

@@ -59,6 +59,7 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
         ens.set_state(np.concatenate(pos), np.concatenate(vel))
         kernel, tile, _ = ens.autotune(10)
         classes = ens.class_choices()
+        merged = ens.merged
         if not eager:
             ens.set_graph(graph_steps)
         ens.step(warmup)
@@ -122,6 +123,7 @@ def run(nrep: int = 512, steps: int = 500, baseline_steps: int = 20, graph_steps
         "speedup_vs_graph_baseline": batched / single_graph if single_graph else None,
         "autotuned_kernel": kernel,
         "autotuned_tile": tile,
+        "one_pair_launch_for_all": merged,
         "occupancy_classes": [{"particles_per_cell": lim, "replicas": n, "kernel": k, "tile": t}
                               for lim, (n, k, t) in zip(("<0.25", "<0.5", "<1", ">=1"),
                                                         classes)],

@@ -267,7 +267,10 @@ struct psim_ens {
     bool brick = false;  // initial pair kernel of every class: bricks
     int block = 256;     // initial workgroup size / brick target
     int target = 192;
-    EnsClass cls[kEnsClasses];
+    // classes 0..kEnsClasses-1 by occupancy; cls[kEnsClasses] holds every
+    // replica (one pair launch for all), used when `merged`
+    EnsClass cls[kEnsClasses + 1];
+    bool merged = false;
     int integ_nblk = 0;        // integrator: 256-lane workgroups over all replicas
     int *integ_blk0 = nullptr; // prefix of the replicas' workgroup counts
     std::vector<RepGeo> h_reps;
@@ -369,7 +372,10 @@ void launch_ens_pair_class(psim_ens *e, const EnsClass &cl, const StepArgs &a)
 template <int MASK>
 void launch_ens_pair_block(psim_ens *e, const StepArgs &a)
 {
-    for (const EnsClass &cl : e->cls) launch_ens_pair_class<MASK>(e, cl, a);
+    if (e->merged)
+        launch_ens_pair_class<MASK>(e, e->cls[kEnsClasses], a);
+    else
+        for (int c = 0; c < kEnsClasses; ++c) launch_ens_pair_class<MASK>(e, e->cls[c], a);
     const int per = (e->integ_nblk + 7) / 8;
     k_ens_integ<256><<<8 * per, 256, 0, e->stream>>>(e->reps, e->p0, e->integ_blk0, e->nrep,
                                                      e->integ_nblk, per, e->ntot, e->gb, a);
@@ -659,8 +665,9 @@ int psim_ens_create(const psim_params *common, const psim_replica *reps, int nre
     for (int r = 0; r < nrep; ++r) {
         e->h_occ_n[r] = reps[r].n_particles;
         e->cls[occ_class((double)reps[r].n_particles / h[r].g.ncell)].reps.push_back(r);
+        e->cls[kEnsClasses].reps.push_back(r);
     }
-    for (int c = 0; c < kEnsClasses && !rc; ++c) {
+    for (int c = 0; c <= kEnsClasses && !rc; ++c) {
         EnsClass &cl = e->cls[c];
         if (cl.reps.empty()) continue;
         const size_t m = cl.reps.size();
@@ -698,7 +705,7 @@ void psim_ens_destroy(psim_ens *e)
                     e->st_hash, e->prev};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    for (EnsClass &cl : e->cls) {
+    for (EnsClass &cl : e->cls) {   // the occupancy classes and the all-replica one
         void *cb[] = {cl.rmap, cl.blk0, cl.item0};
         for (void *b : cb)
             if (b) (void)hipFree(b);
@@ -851,7 +858,9 @@ int psim_ens_pair_stats(psim_ens *e, int *count_by_orig, unsigned long long *has
         if (rc) return rc;
     }
     const NbrStatsOut so{e->st_count, e->st_hash};
-    for (const EnsClass &cl : e->cls) {
+    for (int c = 0; c <= kEnsClasses; ++c) {
+        if ((c == kEnsClasses) != e->merged) continue;
+        const EnsClass &cl = e->cls[c];
         if (cl.reps.empty()) continue;
         const int m = (int)cl.reps.size();
         if (cl.brick) {
@@ -881,14 +890,17 @@ int psim_ens_pair_stats(psim_ens *e, int *count_by_orig, unsigned long long *has
     return 0;
 }
 
-// For each occupancy class: time every candidate pair kernel (three workgroup
-// sizes of the thread-per-particle kernel, three brick sizes of the brick
-// kernel) on that class's replicas alone, trial_steps launches after one
-// warm-up (the pair kernels only write forces, so they repeat on one state),
-// and keep the fastest.  Then trial_steps whole steps with the choices give
-// *best_ms; *kernel and *tile report the choice of the class holding the most
-// particles (psim_ens_class_info gives every class's).  The whole steps
-// advance the ensemble like any other steps; no choice changes a result.
+// For each occupancy class, and for all replicas as one launch: time every
+// candidate pair kernel (three workgroup sizes of the thread-per-particle
+// kernel, three brick sizes of the brick kernel) on those replicas alone,
+// trial_steps launches after one warm-up (the pair kernels only write forces,
+// so they repeat on one state), and keep the fastest.  Then 1 + trial_steps
+// whole steps with one launch per class and as many with one launch for all
+// decide between the two (the separate launches' tails can cost more than
+// the per-class tiles save); *best_ms is the chosen one's.  *kernel and
+// *tile report the choice for the class holding the most particles
+// (psim_ens_class_info gives every class's).  The whole steps advance the
+// ensemble like any other steps; no choice changes a result.
 int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_kernel, int *chosen_tile,
                       float *best_ms)
 {
@@ -900,7 +912,9 @@ int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_kernel, int *cho
     int big = -1;
     long long big_n = -1;
     const StepArgs a = ens_args(e);
-    for (int c = 0; c < kEnsClasses && !rc; ++c) {
+    // each class's candidates on its own, and the same for all replicas in one
+    // launch (class kEnsClasses)
+    for (int c = 0; c <= kEnsClasses && !rc; ++c) {
         EnsClass &cl = e->cls[c];
         if (cl.reps.empty()) continue;
         float best = 0.f;
@@ -925,7 +939,9 @@ int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_kernel, int *cho
                 best_brick = brick;
             }
         }
-        if (!rc) rc = best_brick ? set_bricks(e, c, best_tile) : set_blocks(e, c, best_tile);
+        cl.brick = best_brick;
+        (best_brick ? cl.target : cl.block) = best_tile;
+        if (c == kEnsClasses) continue;
         long long nc = 0;
         for (int r : cl.reps) nc += e->h_occ_n[r];
         if (nc > big_n) {
@@ -933,34 +949,54 @@ int psim_ens_autotune(psim_ens *e, int trial_steps, int *chosen_kernel, int *cho
             big = c;
         }
     }
-    drop_graph(e);   // recaptured with the chosen kernels on the next step
-    float ms = 0.f;
-    if (!rc) rc = enqueue_step(e);
-    if (!rc) {
+    // the replica table holds one brick cut per replica: (re)make the one of
+    // the configuration about to run
+    auto apply = [&](bool merged) {
+        e->merged = merged;
+        int r2 = 0;
+        for (int c = 0; c <= kEnsClasses && !r2; ++c) {
+            EnsClass &cl = e->cls[c];
+            if (cl.reps.empty() || (c == kEnsClasses) != merged) continue;
+            r2 = cl.brick ? set_bricks(e, c, cl.target) : set_blocks(e, c, cl.block);
+        }
+        return r2;
+    };
+    // whole steps per class and in one launch; keep the faster
+    float ms_cls = 0.f, ms_all = 0.f;
+    for (int m = 0; m < 2 && !rc; ++m) {
+        rc = apply(m == 1);
+        if (!rc) rc = enqueue_step(e);
+        if (rc) break;
         ECK(hipEventRecord(t0, e->stream));
         for (int k = 0; k < trial_steps && !rc; ++k) rc = enqueue_step(e);
         ECK(hipEventRecord(t1, e->stream));
         ECK(hipEventSynchronize(t1));
-        ECK(hipEventElapsedTime(&ms, t0, t1));
+        ECK(hipEventElapsedTime(m ? &ms_all : &ms_cls, t0, t1));
     }
+    if (!rc) rc = apply(ms_all < ms_cls);
+    drop_graph(e);   // recaptured with the chosen kernels on the next step
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (rc) return rc;
-    const EnsClass &cb = e->cls[big];
+    const EnsClass &cb = e->cls[e->merged ? kEnsClasses : big];
     if (chosen_kernel) *chosen_kernel = cb.brick ? 3 : 1;
     if (chosen_tile) *chosen_tile = cb.brick ? cb.target : cb.block;
-    if (best_ms) *best_ms = ms / trial_steps;
+    if (best_ms) *best_ms = (e->merged ? ms_all : ms_cls) / trial_steps;
     return 0;
 }
 
 int psim_ens_class_info(psim_ens *e, int cls, int *nrep, int *kernel, int *tile)
 {
     if (!e || cls < 0 || cls >= kEnsClasses) return PSIM_E_ARG;
-    const EnsClass &cl = e->cls[cls];
-    if (nrep) *nrep = (int)cl.reps.size();
+    // the kernel that runs the class's replicas: its own, or the one launch
+    // for all replicas when the autotune found that faster
+    const EnsClass &cl = e->cls[e->merged ? kEnsClasses : cls];
+    if (nrep) *nrep = (int)e->cls[cls].reps.size();
     if (kernel) *kernel = cl.brick ? 3 : 1;
     if (tile) *tile = cl.brick ? cl.target : cl.block;
     return 0;
 }
+
+int psim_ens_merged(psim_ens *e) { return e ? (e->merged ? 1 : 0) : PSIM_E_ARG; }
 
 }  // extern "C"

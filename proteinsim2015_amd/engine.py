@@ -728,8 +728,18 @@ class Ensemble:
                                          C.byref(ms)), "psim_ens_autotune")
         return ("brick" if k.value == 3 else "sparse"), t.value, ms.value
 
+    @property
+    def merged(self) -> bool:
+        """True when one pair launch for all replicas beat one per class."""
+        st = self.lib.psim_ens_merged(self._ctx)
+        if st < 0:
+            check(st, "psim_ens_merged")
+        return bool(st)
+
     def class_choices(self):
-        """[(replicas, kernel, tile)] for each occupancy class, in order."""
+        """[(replicas, kernel, tile)] for each occupancy class, in order (the
+        kernel that runs the class: its own, or the single launch when
+        ``merged``)."""
         out = []
         for c in range(_lib.ENS_CLASSES):
             n, k, t = C.c_int(), C.c_int(), C.c_int()

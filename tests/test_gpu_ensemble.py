@@ -74,7 +74,8 @@ def test_graph_replay_and_autotune_are_bit_identical():
                 ens.set_graph(5)
                 ens.step(12)
             elif mode == "tuned":
-                # the tuning's whole steps: one warm-up and one timed
+                # the tuning's whole steps: one warm-up and one timed, for
+                # one launch per class and for one launch in all (4)
                 kernel, tile, ms = ens.autotune(1)
                 allowed = [("sparse", 64), ("sparse", 128), ("sparse", 256),
                            ("brick", 96), ("brick", 192), ("brick", 384)]
@@ -83,7 +84,8 @@ def test_graph_replay_and_autotune_are_bit_identical():
                 choices = ens.class_choices()
                 assert sum(c[0] for c in choices) == len(reps)
                 assert all((k, t) in allowed for n, k, t in choices if n)
-                ens.step(10)
+                assert ens.merged in (True, False)
+                ens.step(8)
             else:
                 ens.step(12)
             ens.sync()
@@ -126,7 +128,7 @@ def test_full_512_replica_ensemble(oracle):
     print(f"ensemble 512: {len(pos)} particles", flush=True)
     with ps.Ensemble(reps, integ) as ens:
         ens.set_state(pos, vel)
-        kernel, tile, _ = ens.autotune(1)      # whole steps: one warm-up, one timed
+        kernel, tile, _ = ens.autotune(1)      # whole steps: 2 x (one warm-up, one timed)
         ens.set_graph(4)
         ens.step(8)
         ens.sync()
@@ -134,7 +136,7 @@ def test_full_512_replica_ensemble(oracle):
         out = ens.get_state()
         cnt, hs = ens.pair_stats()
         offsets = ens.offsets
-    nsteps = 2 + 8
+    nsteps = 4 + 8
     print(f"ensemble 512: kernel {kernel} tile {tile}", flush=True)
     phis = np.array([r.geometry.volume_fraction for r in reps])
     sizes = np.array([r.geometry.n for r in reps])
