@@ -293,38 +293,75 @@ __global__ void k_move(int n, const int *__restrict__ perm, const int *__restric
 // pos2 / id2 set, the new positions and ids are also written there (a
 // record of the state, e.g. the adjoint's recomputed trajectory), and with
 // inv2 the slot of each original index.
-__global__ void k_move_lagged(int n, const int *__restrict__ perm, const int *__restrict__ key,
-                              const int *__restrict__ cstart, const double *__restrict__ npos,
-                              const int *__restrict__ id_in, double *__restrict__ pos,
-                              int *__restrict__ id_out, int *__restrict__ prev,
-                              StepCtl *ctl, double *__restrict__ pos2 = nullptr,
-                              int *__restrict__ id2 = nullptr, int *__restrict__ inv2 = nullptr)
+//
+// Each lane moves U particles (slots t, t + blockDim.x, ... of its block's
+// U * blockDim.x), their loads interleaved: the kernel is bound by the
+// latency of its chain of dependent gathers (slot -> particle -> cell ->
+// the cell's members), not by its bytes.
+#ifndef PSIM_MOVE_ILP
+#define PSIM_MOVE_ILP 4
+#endif
+constexpr int kMoveIlp = PSIM_MOVE_ILP;
+template <int U>
+__global__ __launch_bounds__(256) void k_move_lagged(
+    int n, const int *__restrict__ perm, const int *__restrict__ key,
+    const int *__restrict__ cstart, const double *__restrict__ npos,
+    const int *__restrict__ id_in, double *__restrict__ pos, int *__restrict__ id_out,
+    int *__restrict__ prev, StepCtl *ctl, double *__restrict__ pos2 = nullptr,
+    int *__restrict__ id2 = nullptr, int *__restrict__ inv2 = nullptr)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ctl && t == 0) ctl_advance(ctl);
-    if (t >= n) return;
-    const int s = perm[t];
-    const int c = key[s];
-    const int b = cstart[c], e = cstart[c + 1];
-    int rank = 0;
-    int u = b;
-    for (; u + 4 <= e; u += 4) {
-        const int p0 = perm[u], p1 = perm[u + 1], p2 = perm[u + 2], p3 = perm[u + 3];
-        rank += (p0 < s) + (p1 < s) + (p2 < s) + (p3 < s);
-    }
-    for (; u < e; ++u) rank += (perm[u] < s);
-    const int dst = b + rank;
+    const int t0 = blockIdx.x * (blockDim.x * U) + threadIdx.x;
+    if (ctl && t0 == 0) ctl_advance(ctl);
+    if (t0 >= n) return;
+    int s[U], b[U], len[U], rank[U];
+    bool ok[U];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double x = npos[d * n + s];
-        pos[d * n + dst] = x;
-        if (pos2) pos2[d * n + dst] = x;
+    for (int u = 0; u < U; ++u) {
+        const int t = t0 + u * (int)blockDim.x;
+        ok[u] = t < n;
+        s[u] = perm[ok[u] ? t : t0];   // a missing slot repeats the first
     }
-    const int o = id_in[s];
-    id_out[dst] = o;
-    if (id2) id2[dst] = o;
-    if (inv2) inv2[o] = dst;
-    prev[dst] = s;
+    int most = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int c = key[s[u]];
+        b[u] = cstart[c];
+        len[u] = cstart[c + 1] - b[u];
+        rank[u] = 0;
+        most = max(most, len[u]);
+    }
+    // rank among the cell's members: four independent loads per particle
+    // per round
+    for (int k = 0; k < most; k += 4) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (k + q < len[u]) rank[u] += (perm[b[u] + k + q] < s[u]);
+        }
+    }
+    double x[U][3];
+    int o[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[u][d] = npos[d * n + s[u]];
+        o[u] = id_in[s[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+        const int dst = b[u] + rank[u];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            pos[d * n + dst] = x[u][d];
+            if (pos2) pos2[d * n + dst] = x[u][d];
+        }
+        id_out[dst] = o[u];
+        if (id2) id2[dst] = o[u];
+        if (inv2) inv2[o[u]] = dst;
+        prev[dst] = s[u];
+    }
 }
 
 // The lagged arrays (previous positions, last forces, memory, kept in the

@@ -102,10 +102,9 @@ int launch_build(psim_ctx *c, StepCtl *advance)
     // positions and ids move into the new order; the previous positions,
     // forces and memory stay where this step wrote them (npos0 / nfrc /
     // nmem, old order) and the next step reads them through prev
-    k_move_lagged<<<blocks_for(n, 256), 256, 0, st>>>(n, c->perm, c->key, c->cstart, c->npos,
-                                                      c->id, c->pos, c->nid, c->prev,
-                                                      advance, c->rec.pos, c->rec.id,
-                                                      c->rec.inv);
+    k_move_lagged<kMoveIlp><<<blocks_for(n, 256 * kMoveIlp), 256, 0, st>>>(
+        n, c->perm, c->key, c->cstart, c->npos, c->id, c->pos, c->nid, c->prev, advance,
+        c->rec.pos, c->rec.id, c->rec.inv);
     CK(hipGetLastError());
     std::swap(c->id, c->nid);
     std::swap(c->pos0, c->npos0);

@@ -395,8 +395,8 @@ void launch_build_ens(psim_ens *e, StepCtl *advance)
                                                direct ? e->count : nullptr, nc);
     // positions and ids move into the new order; the previous positions,
     // forces and memory stay where this step wrote them, read through prev
-    k_move_lagged<<<eblocks(n, 256), 256, 0, st>>>(n, e->perm, e->key, e->cstart, e->npos,
-                                                   e->id, e->pos, e->nid, e->prev, advance);
+    k_move_lagged<kMoveIlp><<<eblocks(n, 256 * kMoveIlp), 256, 0, st>>>(
+        n, e->perm, e->key, e->cstart, e->npos, e->id, e->pos, e->nid, e->prev, advance);
     std::swap(e->id, e->nid);
     std::swap(e->pos0, e->npos0);
     std::swap(e->frc, e->nfrc);
