@@ -366,17 +366,22 @@ __global__ __launch_bounds__(256) void k_move_lagged(
 
 // The lagged arrays (previous positions, last forces, memory, kept in the
 // order of the step that wrote them) gathered into the current slot order.
+// With pos set, pos0 holds the positions the last step started from and is
+// re-imaged against the current positions (util::safeMod0, as the step would
+// have stored it).
 __global__ void k_settle(int n, const int *__restrict__ prev, const double *__restrict__ pos0,
                          const double *__restrict__ frc, const double *__restrict__ mem,
                          double *__restrict__ pos0_out, double *__restrict__ frc_out,
-                         double *__restrict__ mem_out)
+                         double *__restrict__ mem_out, const double *__restrict__ pos = nullptr,
+                         double L = 0.0)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int p = prev[s];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        pos0_out[d * n + s] = pos0[d * n + p];
+        const double x0 = pos0[d * n + p];
+        pos0_out[d * n + s] = pos ? safe_mod0(x0, pos[d * n + s], L) : x0;
         frc_out[d * n + s] = frc[d * n + p];
         mem_out[d * n + s] = mem[d * n + p];
     }

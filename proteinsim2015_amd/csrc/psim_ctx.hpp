@@ -38,9 +38,11 @@ struct psim_ctx {
     double *npos = nullptr, *npos0 = nullptr, *nfrc = nullptr, *nmem = nullptr;
     double *vel = nullptr, *fout = nullptr;
     int *id = nullptr, *nid = nullptr, *key = nullptr, *arank = nullptr, *perm = nullptr;
-    // pos0 / frc / mem stay in the slot order of the step that wrote them;
-    // prev[s] is the slot in that order of the particle now in slot s
-    // (lagged), until settle() gathers them into the current order
+    // pos0 / frc / mem stay in the slot order of the last step: pos0 the
+    // positions that step started from (not yet re-imaged), frc and mem what
+    // it wrote; prev[s] is the slot in that order of the particle now in
+    // slot s (lagged), until settle() gathers them into the current order,
+    // pos0 re-imaged against pos (npos0 is settle()'s staging buffer)
     int *prev = nullptr;
     bool lagged = false;
     // graph replay of graph_steps steps (0: eager launches); ctl is the

@@ -22,8 +22,10 @@
 //   k_scatter   slot of each particle in its new cell (arbitrary order).
 //   k_move_lagged  canonical within-cell order (ascending previous slot,
 //               i.e. a stable counting sort); positions and ids move into
-//               it, the previous positions, forces and memory stay where
-//               the step wrote them (read through the slot map prev).
+//               it; the positions the step started from (pos0 of the next
+//               step, re-imaged when read), the forces and the memory stay
+//               where the step had or wrote them (read through the slot map
+//               prev).
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -99,15 +101,17 @@ int launch_build(psim_ctx *c, StepCtl *advance)
     }
     k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, c->key, c->arank, c->cstart, c->perm,
                                                   direct ? c->count : nullptr, nc);
-    // positions and ids move into the new order; the previous positions,
-    // forces and memory stay where this step wrote them (npos0 / nfrc /
-    // nmem, old order) and the next step reads them through prev
+    // positions and ids move into the new order, the positions into the
+    // buffer of the last step's start positions (read by this step's
+    // integrator, not after it); the positions this step started from become
+    // pos0, and they, the forces and the memory stay where this step had or
+    // wrote them (old order): the next step reads them through prev
     k_move_lagged<kMoveIlp><<<blocks_for(n, 256 * kMoveIlp), 256, 0, st>>>(
-        n, c->perm, c->key, c->cstart, c->npos, c->id, c->pos, c->nid, c->prev, advance,
+        n, c->perm, c->key, c->cstart, c->npos, c->id, c->pos0, c->nid, c->prev, advance,
         c->rec.pos, c->rec.id, c->rec.inv);
     CK(hipGetLastError());
     std::swap(c->id, c->nid);
-    std::swap(c->pos0, c->npos0);
+    std::swap(c->pos, c->pos0);
     std::swap(c->frc, c->nfrc);
     std::swap(c->mem, c->nmem);
     c->lagged = true;
@@ -118,7 +122,8 @@ int settle(psim_ctx *c)
 {
     if (!c->lagged) return 0;
     k_settle<<<blocks_for(c->n, 256), 256, 0, c->stream>>>(c->n, c->prev, c->pos0, c->frc,
-                                                           c->mem, c->npos0, c->nfrc, c->nmem);
+                                                           c->mem, c->npos0, c->nfrc, c->nmem,
+                                                           c->pos, c->prm.box);
     CK(hipGetLastError());
     // copied back rather than swapped: the buffers stay where a captured
     // step sequence expects them
@@ -255,6 +260,7 @@ StepArgs step_args(psim_ctx *c, const double *noise, const StepCtl *ctl)
     a.pos = c->pos; a.pos0 = c->pos0; a.frc = c->frc; a.mem = c->mem;
     a.id = c->id; a.cstart = c->cstart;
     a.prev = c->lagged ? c->prev : nullptr;
+    a.x0_raw = c->lagged ? 1 : 0;
     a.npos = c->npos; a.npos0 = c->npos0; a.nfrc = c->nfrc; a.nmem = c->nmem;
     a.nid = nullptr; a.vel = c->vel; a.key = c->key; a.arank = c->arank;
     a.count = c->count; a.err = c->err;
@@ -507,6 +513,16 @@ int set_state(psim_ctx *c, const double *pos, const double *pos0, const double *
     CK(hipGetLastError());
     rc = launch_build(c);
     if (rc) return rc;
+    // the uploaded previous positions (already re-imaged, staging order),
+    // forces and memory gathered into the slot order; the state is then
+    // settled
+    k_settle<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, c->prev, c->npos0, c->frc, c->mem,
+                                                        c->pos0, c->nfrc, c->nmem);
+    CK(hipGetLastError());
+    const size_t b3 = sizeof(double) * 3 * (size_t)n;
+    CK(hipMemcpyAsync(c->frc, c->nfrc, b3, hipMemcpyDeviceToDevice, c->stream));
+    CK(hipMemcpyAsync(c->mem, c->nmem, b3, hipMemcpyDeviceToDevice, c->stream));
+    c->lagged = false;
     CK(hipStreamSynchronize(c->stream));
     c->time = current_time;
     c->vel_counter = vel_counter;

@@ -72,6 +72,8 @@ struct StepArgs {
     const double *pos, *pos0, *frc, *mem;
     const int *id, *cstart;
     const int *prev;     // slot of each particle in pos0 / frc / mem (null: the same slot)
+    int x0_raw;          // pos0 holds the positions the last step started from (re-imaged
+                         // here against the current ones); npos0 is then not written
     double *npos, *npos0, *nfrc, *nmem;
     int *nid;            // copy of id in the current order (null: not kept)
     double *vel;         // [3][N] by original index
@@ -115,6 +117,12 @@ __device__ __forceinline__ void integrate_one(const Geo &g, const GB &gb, const 
     double x0c[3], xn[3], M[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) x0c[d] = a.pos0[d * N + ps];
+    if (a.x0_raw) {
+        // what the last step would have stored: util::safeMod0 of its start
+        // position against its end position, which is this step's start
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x0c[d] = safe_mod0(x0c[d], xc[d], g.L);
+    }
     if (first) {
         // firstStep :200-218
 #pragma unroll
@@ -177,7 +185,7 @@ __device__ __forceinline__ void integrate_one(const Geo &g, const GB &gb, const 
                 a.pbc[(size_t)d * N + o] -= 1;
         }
         a.npos[d * N + s] = xw[d];
-        a.npos0[d * N + s] = x0n;
+        if (!a.x0_raw) a.npos0[d * N + s] = x0n;
         if (WRITE_F) a.nfrc[d * N + s] = F[d];
         a.nmem[d * N + s] = M[d];
     }
