@@ -223,6 +223,28 @@ __global__ void k_iota_ens(int n, int *p)
     if (s < n) p[s] = s;
 }
 
+// The lagged arrays gathered into the current slot order (k_settle), pos0
+// (the positions the last step started from) re-imaged against pos in the
+// box of the slot's replica (slots p0[r]..p0[r+1] belong to replica r).
+__global__ void k_settle_ens(int n, const RepGeo *__restrict__ reps,
+                             const int *__restrict__ p0, int nrep,
+                             const int *__restrict__ prev, const double *__restrict__ pos0,
+                             const double *__restrict__ frc, const double *__restrict__ mem,
+                             const double *__restrict__ pos, double *__restrict__ pos0_out,
+                             double *__restrict__ frc_out, double *__restrict__ mem_out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int p = prev[s];
+    const double L = reps[replica_of(p0, nrep, s)].g.L;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        pos0_out[d * n + s] = safe_mod0(pos0[d * n + p], pos[d * n + s], L);
+        frc_out[d * n + s] = frc[d * n + p];
+        mem_out[d * n + s] = mem[d * n + p];
+    }
+}
+
 // Initial hash of every particle into its replica's cells.
 __global__ void k_ens_hash(const RepGeo *__restrict__ reps, const int *__restrict__ p0,
                            int nrep, int ntot, const double *pos, const int *id, int *key,
@@ -285,8 +307,9 @@ struct psim_ens {
     double *vel = nullptr;
     int *id = nullptr, *nid = nullptr, *key = nullptr, *arank = nullptr, *perm = nullptr;
     int *count = nullptr, *cstart = nullptr, *bsum = nullptr;
-    // pos0 / frc / mem stay in the slot order of the step that wrote them;
-    // prev[s] is the slot in that order of the particle now in slot s
+    // pos0 / frc / mem stay in the slot order of the last step (pos0: the
+    // positions it started from, re-imaged when read); prev[s] is the slot
+    // in that order of the particle now in slot s
     int *prev = nullptr;
     bool lagged = false;
     DevErr *err = nullptr;
@@ -333,6 +356,7 @@ StepArgs ens_args(psim_ens *e)
     a.pos = e->pos; a.pos0 = e->pos0; a.frc = e->frc; a.mem = e->mem;
     a.id = e->id; a.cstart = e->cstart;
     a.prev = e->lagged ? e->prev : nullptr;
+    a.x0_raw = e->lagged ? 1 : 0;
     a.npos = e->npos; a.npos0 = e->npos0; a.nfrc = e->nfrc; a.nmem = e->nmem;
     a.nid = nullptr; a.vel = e->vel; a.key = e->key; a.arank = e->arank;
     a.count = e->count; a.err = e->err;
@@ -381,6 +405,15 @@ void launch_ens_pair_block(psim_ens *e, const StepArgs &a)
                                                      e->integ_nblk, per, e->ntot, e->gb, a);
 }
 
+// the buffer roles after one step's cell rebuild
+void swap_lagged(psim_ens *e)
+{
+    std::swap(e->id, e->nid);
+    std::swap(e->pos, e->pos0);
+    std::swap(e->frc, e->nfrc);
+    std::swap(e->mem, e->nmem);
+}
+
 void launch_build_ens(psim_ens *e, StepCtl *advance)
 {
     const int n = e->ntot, nc = e->ncell;
@@ -393,14 +426,14 @@ void launch_build_ens(psim_ens *e, StepCtl *advance)
     }
     k_scatter<<<eblocks(n, 256), 256, 0, st>>>(n, e->key, e->arank, e->cstart, e->perm,
                                                direct ? e->count : nullptr, nc);
-    // positions and ids move into the new order; the previous positions,
-    // forces and memory stay where this step wrote them, read through prev
+    // positions and ids move into the new order, the positions into the
+    // buffer of the last step's start positions (read by this step's
+    // integrator, not after it); the positions this step started from become
+    // pos0, and they, the forces and the memory stay where this step had or
+    // wrote them: the next step reads them through prev
     k_move_lagged<kMoveIlp><<<eblocks(n, 256 * kMoveIlp), 256, 0, st>>>(
-        n, e->perm, e->key, e->cstart, e->npos, e->id, e->pos, e->nid, e->prev, advance);
-    std::swap(e->id, e->nid);
-    std::swap(e->pos0, e->npos0);
-    std::swap(e->frc, e->nfrc);
-    std::swap(e->mem, e->nmem);
+        n, e->perm, e->key, e->cstart, e->npos, e->id, e->pos0, e->nid, e->prev, advance);
+    swap_lagged(e);
     e->lagged = true;
 }
 
@@ -410,8 +443,9 @@ int settle_ens(psim_ens *e)
 {
     if (!e->lagged) return 0;
     const int n = e->ntot;
-    k_settle<<<eblocks(n, 256), 256, 0, e->stream>>>(n, e->prev, e->pos0, e->frc, e->mem,
-                                                     e->npos0, e->nfrc, e->nmem);
+    k_settle_ens<<<eblocks(n, 256), 256, 0, e->stream>>>(n, e->reps, e->p0, e->nrep, e->prev,
+                                                         e->pos0, e->frc, e->mem, e->pos,
+                                                         e->npos0, e->nfrc, e->nmem);
     ECK(hipGetLastError());
     const size_t b = sizeof(double) * 3 * (size_t)n;
     ECK(hipMemcpyAsync(e->pos0, e->npos0, b, hipMemcpyDeviceToDevice, e->stream));
@@ -546,12 +580,7 @@ int graph_for_now(psim_ens *e, hipGraphExec_t *out)
         if (e->exec[k] && e->gkey[k] == key) {
             *out = e->exec[k];
             // the buffers the replay leaves behind
-            for (int q = 0; q < e->graph_steps; ++q) {
-                std::swap(e->id, e->nid);
-                std::swap(e->pos0, e->npos0);
-                std::swap(e->frc, e->nfrc);
-                std::swap(e->mem, e->nmem);
-            }
+            for (int q = 0; q < e->graph_steps; ++q) swap_lagged(e);
             return 0;
         }
     const int slot = e->exec[0] ? 1 : 0;
@@ -769,6 +798,15 @@ int psim_ens_set_state(psim_ens *e, const double *pos, const double *vel, double
                                                        e->id, e->key, e->arank, e->count);
     launch_build_ens(e, nullptr);
     ECK(hipGetLastError());
+    // the fresh previous positions (already re-imaged, upload order), forces
+    // and memory gathered into the slot order; the state is then settled
+    k_settle<<<eblocks(n, 256), 256, 0, e->stream>>>(n, e->prev, e->npos0, e->frc, e->mem,
+                                                     e->pos0, e->nfrc, e->nmem);
+    ECK(hipGetLastError());
+    const size_t b3 = sizeof(double) * 3 * (size_t)n;
+    ECK(hipMemcpyAsync(e->frc, e->nfrc, b3, hipMemcpyDeviceToDevice, e->stream));
+    ECK(hipMemcpyAsync(e->mem, e->nmem, b3, hipMemcpyDeviceToDevice, e->stream));
+    e->lagged = false;
     ECK(hipStreamSynchronize(e->stream));
     e->have_state = true;
     return 0;
