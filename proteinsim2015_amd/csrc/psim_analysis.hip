@@ -17,6 +17,7 @@
 // g(r)); it is the observable of the structure-fitting loss.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -183,6 +184,12 @@ __global__ void k_cc_jump(int n, int *lab)
 }
 
 // ---- pair-distance histogram -----------------------------------------------
+// Gauss–Legendre points per sub-interval of a bin (sigma > 0): with
+// sub-intervals of at most 1.25 sigma the rule matches the CDF difference to
+// the last bits (2.2e-16 at 1 sigma, 7.8e-16 at 1.25 against scipy's ndtr).
+constexpr int kGLNodes = 8;
+constexpr double kGLMaxStep = 1.25;
+
 struct HistGeo {
     int n;
     int S, S2;          // histogram grid (S >= 3), or S == 0 for all pairs
@@ -190,6 +197,12 @@ struct HistGeo {
     double r_lo, width, sigma, inv_sigma, reach2;
     double delta;       // width / sigma
     int nbins;
+    // bin masses: gl_m sub-intervals of gl_h = delta / gl_m per bin, each by
+    // the kGLNodes-point Gauss–Legendre rule (nodes gl_x in [0, 1], weights
+    // gl_w summing to 1); gl_c = e^{-h^2}, gl_a[k] = e^{-h^2 (x_k + 1/2)}
+    int gl_m;
+    double gl_h, gl_c;
+    double gl_x[kGLNodes], gl_w[kGLNodes], gl_a[kGLNodes];
 };
 
 __device__ inline double norm_cdf(double t) { return 0.5 * erfc(-t * 0.70710678118654752440); }
@@ -237,6 +250,53 @@ __device__ inline void edge_window(const HistGeo &h, double r, int &k0, int &k1)
     k1 = (int)fmin((double)h.nbins, floor(hi));
 }
 
+// The Gaussian's mass (mean r, width sigma > 0) in each bin it reaches,
+// handed to add(bin, mass): bin k0 - 1 takes the mass below edge k0 (the CDF
+// there), bins k0 .. k1 - 1 the mass between their edges, bin k1 the mass
+// above edge k1 (k0, k1 from edge_window), as the CDF differences at the
+// edges would give them.  The mass between edges is the Gauss–Legendre rule
+// over gl_m sub-intervals, with the density at each node carried from one
+// sub-interval to the next by phi(u + h) = phi(u) e^{-h u - h^2/2}: one CDF
+// and kGLNodes + 1 exponentials per pair instead of one CDF per edge.
+template <class Add>
+__device__ __forceinline__ void pair_masses(const HistGeo &h, double r, Add add)
+{
+    int k0, k1;
+    edge_window(h, r, k0, k1);
+    const int nb = h.nbins;
+    if (k0 > k1) {
+        if (k1 >= 0 && k1 < nb) add(k1, 1.0);
+        return;
+    }
+    const double t0 = (h.r_lo + h.width * k0 - r) * h.inv_sigma;
+    double E = norm_cdf(t0);   // the mass below edge k0
+    if (k0 - 1 >= 0 && k0 - 1 < nb) add(k0 - 1, E);
+    if (k0 < k1) {
+        double phi[kGLNodes], g[kGLNodes];
+        const double A = exp(-h.gl_h * t0);
+#pragma unroll
+        for (int k = 0; k < kGLNodes; ++k) {
+            phi[k] = norm_pdf(fma(h.gl_h, h.gl_x[k], t0));
+            g[k] = A * h.gl_a[k];
+        }
+        for (int q = k0; q < k1; ++q) {   // bin q: edges q and q + 1
+            double m = 0.0;
+            for (int sub = 0; sub < h.gl_m; ++sub) {
+#pragma unroll
+                for (int k = 0; k < kGLNodes; ++k) {
+                    m = fma(h.gl_w[k], phi[k], m);
+                    phi[k] *= g[k];
+                    g[k] *= h.gl_c;
+                }
+            }
+            m *= h.gl_h;
+            if (q < nb) add(q, m);
+            E += m;
+        }
+    }
+    if (k1 >= 0 && k1 < nb) add(k1, 1.0 - E);
+}
+
 // h_b = sum over ordered pairs i != j of the Gaussian mass of r_ij in bin b
 // (sigma > 0), or the count of r_ij in [e_b, e_b+1) (sigma == 0).
 __global__ __launch_bounds__(kAnaBlock) void k_pair_hist(HistGeo h, const double *pos,
@@ -256,17 +316,7 @@ __global__ __launch_bounds__(kAnaBlock) void k_pair_hist(HistGeo h, const double
                 if (q >= 0.0 && q < (double)h.nbins) atomicAdd(&sh[(int)q], 1.0);
                 return;
             }
-            int k0, k1;
-            edge_window(h, r, k0, k1);
-            double prev = 0.0;   // CDF at the edge below
-            for (int k = k0; k <= k1; ++k) {
-                const double e = h.r_lo + h.width * k;
-                const double E = norm_cdf((e - r) * h.inv_sigma);
-                if (k - 1 >= 0 && k - 1 < h.nbins) atomicAdd(&sh[k - 1], E - prev);
-                prev = E;
-            }
-            if (k1 < h.nbins && k1 >= 0 && k0 <= k1) atomicAdd(&sh[k1], 1.0 - prev);
-            else if (k0 > k1 && k1 >= 0 && k1 < h.nbins) atomicAdd(&sh[k1], 1.0);
+            pair_masses(h, r, [&](int b, double m) { atomicAdd(&sh[b], m); });
         });
     }
     __syncthreads();
@@ -392,17 +442,7 @@ __global__ __launch_bounds__(kAnaBlock) void k_pair_hist_grid(HistGeo h,
     auto drain = [&](int nh) {
         for (int k = 0; k < nh; ++k) {
             const double r = sqrt(lst[k * kAnaBlock + threadIdx.x]);
-            int k0, k1;
-            edge_window(h, r, k0, k1);
-            double prev = 0.0;   // CDF at the edge below
-            for (int q = k0; q <= k1; ++q) {
-                const double e = h.r_lo + h.width * q;
-                const double E = norm_cdf((e - r) * h.inv_sigma);
-                if (q - 1 >= 0 && q - 1 < nb) atomicAdd(&mine[q - 1], E - prev);
-                prev = E;
-            }
-            if (k1 < nb && k1 >= 0 && k0 <= k1) atomicAdd(&mine[k1], 1.0 - prev);
-            else if (k0 > k1 && k1 >= 0 && k1 < nb) atomicAdd(&mine[k1], 1.0);
+            pair_masses(h, r, [&](int b, double m) { atomicAdd(&mine[b], m); });
         }
     };
     int nh = 0;
@@ -554,6 +594,32 @@ int ensure_work(psim_ctx *c, size_t doubles, size_t ints)
     return 0;
 }
 
+// Nodes x (ascending) and weights w (summing to 1) of the n-point
+// Gauss–Legendre rule on [0, 1]: Newton's iteration on the roots of P_n,
+// started from cos(pi (i + 3/4) / (n + 1/2)), w = 2 / ((1 - z^2) P_n'(z)^2)
+// on [-1, 1] (Abramowitz & Stegun 25.4.29).
+void gauss_legendre01(int n, double *x, double *w)
+{
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < n; ++i) {
+        double z = std::cos(pi * (i + 0.75) / (n + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int j = 1; j <= n; ++j) {
+                const double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            dp = n * (z * p1 - p2) / (z * z - 1.0);
+            const double z1 = z;
+            z = z1 - p1 / dp;
+            if (std::fabs(z - z1) <= 1e-16) break;
+        }
+        x[i] = 0.5 * (1.0 - z);
+        w[i] = 1.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
 int check_hist_args(psim_ctx *c, double r_lo, double width, int nbins, double sigma)
 {
     if (!(r_lo >= 0.0) || !(width > 0.0) || nbins <= 0 || nbins > kMaxBins || !(sigma >= 0.0))
@@ -581,6 +647,11 @@ int build_hist_grid(psim_ctx *c, double r_lo, double width, int nbins, double si
     h.delta = width * h.inv_sigma;
     h.reach2 = reach * reach;
     h.nbins = nbins;
+    gauss_legendre01(kGLNodes, h.gl_x, h.gl_w);
+    h.gl_m = sigma > 0.0 ? std::max(1, (int)std::ceil(h.delta / kGLMaxStep)) : 1;
+    h.gl_h = h.delta / h.gl_m;
+    h.gl_c = std::exp(-h.gl_h * h.gl_h);
+    for (int k = 0; k < kGLNodes; ++k) h.gl_a[k] = std::exp(-h.gl_h * h.gl_h * (h.gl_x[k] + 0.5));
     int S = (int)std::floor(L / reach);
     if (S > kMaxHistCells) S = kMaxHistCells;
     if (S < 3) {

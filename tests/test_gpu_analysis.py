@@ -99,6 +99,22 @@ def test_pair_histogram_matches_oracle(oracle, L, S):
     assert np.abs(grad - gref).max() <= 1e-10 * max(1.0, np.abs(gref).max())
 
 
+@pytest.mark.parametrize("w,sigma,nb", [(0.1, 0.02, 30), (0.02, 0.05, 100), (0.05, 0.01, 40)])
+def test_pair_histogram_bin_to_sigma_ratios(oracle, w, sigma, nb):
+    """Bin widths of 5, 0.4 and 5 sigma: the bin masses then take several
+    Gauss-Legendre sub-intervals per bin, or one shorter than sigma."""
+    print(f"histogram w={w} sigma={sigma}", flush=True)
+    L = 16.0
+    sysg, pos, vel = ao_system(600, L, 12)
+    with sysg:
+        assert sysg.step(5) == 0
+        x = sysg.get_state()["pos"]
+        smooth = sysg.pair_histogram(0.8, w, nb, sigma)
+    ref = oracle.pair_histogram_smooth(x, L, 0.8, w, nb, sigma)
+    assert np.abs(smooth - ref).max() <= 1e-10 * max(1.0, ref.max())
+    assert abs(smooth.sum() - ref.sum()) <= 1e-10 * ref.sum()
+
+
 def test_histogram_rejects_reach_beyond_half_box():
     sysg, _, _ = ao_system(80, 6.5, 5)
     with sysg:
