@@ -302,6 +302,9 @@ __global__ void k_move(int n, const int *__restrict__ perm, const int *__restric
 #define PSIM_MOVE_ILP 4
 #endif
 constexpr int kMoveIlp = PSIM_MOVE_ILP;
+// below this many particles the grid is too small to give up lanes for
+// interleaving (one particle per lane)
+constexpr int kMoveIlpMin = 1 << 20;
 template <int U>
 __global__ __launch_bounds__(256) void k_move_lagged(
     int n, const int *__restrict__ perm, const int *__restrict__ key,

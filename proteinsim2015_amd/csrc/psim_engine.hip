@@ -106,9 +106,14 @@ int launch_build(psim_ctx *c, StepCtl *advance)
     // integrator, not after it); the positions this step started from become
     // pos0, and they, the forces and the memory stay where this step had or
     // wrote them (old order): the next step reads them through prev
-    k_move_lagged<kMoveIlp><<<blocks_for(n, 256 * kMoveIlp), 256, 0, st>>>(
-        n, c->perm, c->key, c->cstart, c->npos, c->id, c->pos0, c->nid, c->prev, advance,
-        c->rec.pos, c->rec.id, c->rec.inv);
+    if (n >= kMoveIlpMin)
+        k_move_lagged<kMoveIlp><<<blocks_for(n, 256 * kMoveIlp), 256, 0, st>>>(
+            n, c->perm, c->key, c->cstart, c->npos, c->id, c->pos0, c->nid, c->prev, advance,
+            c->rec.pos, c->rec.id, c->rec.inv);
+    else
+        k_move_lagged<1><<<blocks_for(n, 256), 256, 0, st>>>(
+            n, c->perm, c->key, c->cstart, c->npos, c->id, c->pos0, c->nid, c->prev, advance,
+            c->rec.pos, c->rec.id, c->rec.inv);
     CK(hipGetLastError());
     std::swap(c->id, c->nid);
     std::swap(c->pos, c->pos0);

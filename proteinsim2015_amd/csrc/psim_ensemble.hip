@@ -431,8 +431,12 @@ void launch_build_ens(psim_ens *e, StepCtl *advance)
     // integrator, not after it); the positions this step started from become
     // pos0, and they, the forces and the memory stay where this step had or
     // wrote them: the next step reads them through prev
-    k_move_lagged<kMoveIlp><<<eblocks(n, 256 * kMoveIlp), 256, 0, st>>>(
-        n, e->perm, e->key, e->cstart, e->npos, e->id, e->pos0, e->nid, e->prev, advance);
+    if (n >= kMoveIlpMin)
+        k_move_lagged<kMoveIlp><<<eblocks(n, 256 * kMoveIlp), 256, 0, st>>>(
+            n, e->perm, e->key, e->cstart, e->npos, e->id, e->pos0, e->nid, e->prev, advance);
+    else
+        k_move_lagged<1><<<eblocks(n, 256), 256, 0, st>>>(
+            n, e->perm, e->key, e->cstart, e->npos, e->id, e->pos0, e->nid, e->prev, advance);
     swap_lagged(e);
     e->lagged = true;
 }
