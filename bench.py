@@ -246,9 +246,10 @@ def valu_from_profiles(prof, pair_ms):
             "frac": rate / VALU_PEAK_LANE_OPS,
             "note": "SQ_INSTS_VALU x 64 (issued lanes incl. masked ones) from the committed "
                     "pmc run over this run's launch time, against 1024 SIMDs x 16 lanes x "
-                    "2.4 GHz; the pmc run's own launches (slower under the profiler) put the "
-                    "kernel at 98% of that model, so a frac near or slightly above 1 reads as "
-                    "VALU-issue bound"}
+                    "2.4 GHz; the profiled launches (slower under the profiler) put the "
+                    "kernel at 94-98% of that model, so a frac near or slightly above 1 reads as "
+                    "VALU-issue bound, with the waves' waits binding about as tightly (a cull "
+                    "with 3.6% fewer VALU instructions ran in the same time)"}
 
 
 def small_leg(name: str, steps: int, warmup: int = 10, eager: bool = False) -> dict:
