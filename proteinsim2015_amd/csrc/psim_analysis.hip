@@ -695,7 +695,7 @@ int build_hist_grid(psim_ctx *c, double r_lo, double width, int nbins, double si
     hipStream_t st = c->stream;
     k_hash_count<<<blocks_for(n, 256), 256, 0, st>>>(hg, c->pos, g.key, g.rank, g.count);
     launch_scan3(st, g.count, ncell, g.nscan, g.bsum, g.cstart, n);
-    k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, g.key, g.rank, g.cstart, g.perm + n,
+    k_scatter<1><<<blocks_for(n, 256), 256, 0, st>>>(n, g.key, g.rank, g.cstart, g.perm + n,
                                                   nullptr, 0);
     k_hg_canon<<<blocks_for(n, 256), 256, 0, st>>>(n, g.perm + n, g.key, g.cstart, g.perm,
                                                    c->pos, g.hpos);

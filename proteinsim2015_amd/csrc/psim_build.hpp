@@ -234,15 +234,30 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_direct(const int *count, 
 }
 
 // With clear_count set (after k_scan_direct), also zeroes the ncell counts
-// for the next step's counting.
-__global__ void k_scatter(int n, const int *key, const int *arank, const int *cstart,
-                          int *perm, int *clear_count, int ncell)
+// for the next step's counting.  Each lane places U slots (s, s + blockDim.x,
+// ... of its block's U * blockDim.x), their loads issued together.
+template <int U = 1>
+__global__ __launch_bounds__(256) void k_scatter(int n, const int *key, const int *arank,
+                                                 const int *cstart, int *perm,
+                                                 int *clear_count, int ncell)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (clear_count)
-        for (int c = s; c < ncell; c += gridDim.x * blockDim.x) clear_count[c] = 0;
-    if (s >= n) return;
-    perm[cstart[key[s]] + arank[s]] = s;
+        for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+             c += gridDim.x * blockDim.x)
+            clear_count[c] = 0;
+    const int s0 = blockIdx.x * (blockDim.x * U) + threadIdx.x;
+    int k[U], r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int s = s0 + u * (int)blockDim.x;
+        k[u] = s < n ? key[s] : 0;
+        r[u] = s < n ? arank[s] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int s = s0 + u * (int)blockDim.x;
+        if (s < n) perm[cstart[k[u]] + r[u]] = s;
+    }
 }
 
 // Canonical order within a cell = ascending source slot (stable counting

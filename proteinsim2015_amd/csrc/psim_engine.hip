@@ -99,8 +99,13 @@ int launch_build(psim_ctx *c, StepCtl *advance)
     } else {
         launch_scan3(st, c->count, nc, c->nscan, c->bsum, c->cstart, n, c->rec.cstart);
     }
-    k_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, c->key, c->arank, c->cstart, c->perm,
-                                                  direct ? c->count : nullptr, nc);
+    if (n >= kMoveIlpMin)
+        k_scatter<kMoveIlp><<<blocks_for(n, 256 * kMoveIlp), 256, 0, st>>>(
+            n, c->key, c->arank, c->cstart, c->perm, direct ? c->count : nullptr, nc);
+    else
+        k_scatter<1><<<blocks_for(n, 256), 256, 0, st>>>(n, c->key, c->arank, c->cstart,
+                                                         c->perm, direct ? c->count : nullptr,
+                                                         nc);
     // positions and ids move into the new order, the positions into the
     // buffer of the last step's start positions (read by this step's
     // integrator, not after it); the positions this step started from become

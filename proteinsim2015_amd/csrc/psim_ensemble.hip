@@ -424,8 +424,12 @@ void launch_build_ens(psim_ens *e, StepCtl *advance)
     } else {
         launch_scan3(st, e->count, nc, e->nscan, e->bsum, e->cstart, n);
     }
-    k_scatter<<<eblocks(n, 256), 256, 0, st>>>(n, e->key, e->arank, e->cstart, e->perm,
-                                               direct ? e->count : nullptr, nc);
+    if (n >= kMoveIlpMin)
+        k_scatter<kMoveIlp><<<eblocks(n, 256 * kMoveIlp), 256, 0, st>>>(
+            n, e->key, e->arank, e->cstart, e->perm, direct ? e->count : nullptr, nc);
+    else
+        k_scatter<1><<<eblocks(n, 256), 256, 0, st>>>(n, e->key, e->arank, e->cstart, e->perm,
+                                                      direct ? e->count : nullptr, nc);
     // positions and ids move into the new order, the positions into the
     // buffer of the last step's start positions (read by this step's
     // integrator, not after it); the positions this step started from become

@@ -204,7 +204,7 @@ extern "C" int psim_gpu_forces_noalloc(const psim_params *params, const double *
     } else {
         launch_scan3(st, count, ncell, nscan, bsum, cstart, n);
     }
-    k_scatter<<<eng::blocks_for(n, 256), 256, 0, st>>>(n, key, rank, cstart, perm,
+    k_scatter<1><<<eng::blocks_for(n, 256), 256, 0, st>>>(n, key, rank, cstart, perm,
                                                        direct ? count : nullptr, ncell);
     k_move_aos<<<eng::blocks_for(n, 256), 256, 0, st>>>(n, perm, key, cstart, d_pos, pos, id);
     switch (pot.mask) {
